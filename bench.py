@@ -1,0 +1,153 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the gfx950 TDMP decoder (one JSON line on stdout).
+
+Workload: rate 5/6, z = 96 (n = 2304, k = 1920), 65536 seeded BPSK/AWGN
+frames at Eb/N0 = 4.0 dB generated on the GPU, FP32 LLRs, normalised
+min-sum with exactly 10 iterations (no early stop).  One step = one decode
+of the whole batch (LLRs resident in HBM -> packed hard decisions and
+per-frame status in HBM).  The CPU baseline is the oracle (the C restatement
+of the reference decoder, OpenMP over frames as lib/decode_example.cc does)
+on the same LLR buffer, timed on this host's cores.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--no-cpu]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+RATE, Z, FRAMES, EBN0, ITERS, SEED = 5, 96, 65536, 4.0, 10, 20260101
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--frames", type=int, default=FRAMES)
+    ap.add_argument("--frames-per-wg", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-frames", type=int, default=FRAMES,
+                    help="frames of the same buffer decoded by the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the decoder shards frames across GPUs "
+              "with no exchange, so run one process per GPU (--gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import wimax_ldpc_lib_amd as W
+    if not torch.cuda.is_available() or W.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    F = args.frames
+    enc = W.ldpc_encoder(RATE, Z)
+    dec = W.ldpc_decoder(RATE, Z, ITERS, early_stop=False)
+    if args.frames_per_wg:
+        dec.frames_per_wg = args.frames_per_wg
+    n, k = dec.n, dec.k
+    stream = torch.cuda.current_stream()
+    llr = torch.empty((F, n), dtype=torch.float32, device="cuda")
+    bits = torch.empty((F, n), dtype=torch.uint8, device="cuda")
+    enc.channel_dev(SEED, 0, F, EBN0, llr=llr, bits=bits)
+    hard = torch.empty((F, n // 8), dtype=torch.uint8, device="cuda")
+    ret = torch.empty(F, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    log("frames %d, frames/workgroup %d" % (F, dec.frames_per_wg))
+
+    for _ in range(args.warmup):
+        dec.decode_dev(llr, None, hard, ret)
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        dec.decode_dev(llr, None, hard, ret)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    ms_step = wall * 1e3 / args.steps
+    kavg = sum(kern_ms) / len(kern_ms)
+    value = F * k / (ms_step * 1e-3) / 1e9
+    bytes_per_frame = 4 * n + n // 8
+    achieved = F * bytes_per_frame / (kavg * 1e-3) / 1e9
+    hard_np = hard.cpu().numpy()
+    ber = float(np.mean(np.unpackbits(hard_np[:, :k // 8], axis=1, bitorder="little")
+                        != bits[:, :k].cpu().numpy()))
+    log("kernel avg %.3f ms (min %.3f), step %.3f ms, %.2f info Gbit/s, info BER %.3g"
+        % (kavg, min(kern_ms), ms_step, value, ber))
+
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "tdmp_r5z96_f65536_traffic.json")
+    if os.path.exists(pmc):
+        try:
+            traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        code = O.Code(RATE, Z)
+        S = min(args.cpu_frames, F)
+        sample = llr[:S].cpu().numpy()
+        threads = O.omp_max_threads()
+        warm = code.decode(sample[:256], ITERS, False, nthreads=threads)
+        del warm
+        res = code.decode(sample, ITERS, False, nthreads=threads)
+        cpu_gbs = S * k / res["seconds"] / 1e9
+        same = bool(np.array_equal(np.packbits(res["hd"], axis=1, bitorder="little"), hard_np[:S]))
+        res_es = code.decode(sample, ITERS, True, nthreads=threads)
+        cpu_es = S * k / res_es["seconds"] / 1e9
+        log("cpu baseline (%d threads): fixed-10 %.4f info Gbit/s (%.2f s), early-stop %.4f; "
+            "GPU == CPU hard decisions on %d frames: %s" % (threads, cpu_gbs, res["seconds"], cpu_es, S, same))
+        cpu = {"value": cpu_gbs, "unit": "info Gbit/s", "cores": threads, "kind": "port",
+               "sample": "%d of the same frames, exactly 10 iterations (early-stop as shipped: %.4f "
+                         "info Gbit/s); OpenMP frame loop" % (S, cpu_es),
+               "seconds": res["seconds"], "bit_exact_vs_gpu": same}
+
+    out = {
+        "metric": "info Gbit/s decoded (rate 5/6, z=96, 10 it)",
+        "value": value,
+        "unit": "info Gbit/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "TDMP NMS decode, rate 5/6, z=96, n=2304, k=1920, %d frames, "
+                               "BPSK/AWGN Eb/N0 4.0 dB, exactly 10 iterations" % F,
+                   "frames": F, "frames_per_wg": dec.frames_per_wg, "iterations": ITERS},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "kernel_ms": kavg, "bytes_per_frame": bytes_per_frame},
+        "cpu_baseline": cpu,
+        "info_ber": ber,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,173 @@
+/*
+ * wimax_ldpc_amd.h -- C ABI of the MI355X (gfx950) WiMAX QC-LDPC codec.
+ *
+ * The reference library (myersw12/wimax_ldpc_lib) has no C ABI: its public
+ * interface is the C++ classes in include/ldpc.h, include/ldpc_decoder.h,
+ * include/ldpc_encoder.h and include/BER_runner.h.  This header is the flat
+ * boundary those classes (re-declared with the same signatures in this
+ * directory) are implemented on; it takes plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Soft values use the reference convention: a value >= 0 (including -0.0)
+ *     slices to bit 1 (lib/ldpc_decoder.cc:130,242; bit b is sent as 2b-1 at
+ *     lib/BER_runner.cc:183).
+ *   - Packed hard decisions: bit (v % 8) of byte (v / 8) is variable v.
+ *   - Functions named *_dev take device pointers and enqueue on `stream`
+ *     (a hipStream_t, NULL = the null stream); they do not synchronize.
+ *     Functions named *_host take host pointers and return when done.
+ *   - Every function returns 0 or a negative WL_E* code; wl_last_error()
+ *     describes the last failure of the calling thread.  No C++ exception
+ *     crosses this boundary.
+ *   - A handle may be used by one host thread at a time (as the reference's
+ *     objects: one decoder per thread, lib/decode_example.cc:75-77).
+ */
+#ifndef WIMAX_LDPC_AMD_H
+#define WIMAX_LDPC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WL_OK        0
+#define WL_EINVAL  (-22)   /* bad rate, z, size or pointer */
+#define WL_ENOMEM  (-12)   /* device or host allocation failed */
+#define WL_EDEVICE  (-5)   /* HIP runtime error (no device, launch failure, ...) */
+#define WL_ENOTSUP (-95)   /* valid request this build does not support */
+
+/* coderate values, identical to the reference's enum coderate (include/enums.h:4-12) */
+#define WL_RATE_1_2   0
+#define WL_RATE_2_3A  1
+#define WL_RATE_2_3B  2
+#define WL_RATE_3_4A  3
+#define WL_RATE_3_4B  4
+#define WL_RATE_5_6   5
+
+/* decoder flags */
+#define WL_FLAG_EARLY_STOP  0x1u  /* syndrome check before and after every iteration (as shipped) */
+
+typedef struct wl_decoder wl_decoder;
+typedef struct wl_encoder wl_encoder;
+
+typedef struct {
+    int rate;
+    int z;
+    int n;     /* codeword bits, m_N (lib/ldpc.cc:16) */
+    int m;     /* parity checks, m_M (lib/ldpc.cc:18-45) = m_row_size */
+    int k;     /* info bits n - m */
+    int c;     /* max row degree, m_col_size */
+    int mb;    /* base rows = decoding layers */
+    int d;     /* encoder constant m_D at this z */
+    int x;     /* encoder constant m_X */
+} wl_code_info;
+
+/* Last error text of the calling thread ("" if none). */
+const char* wl_last_error(void);
+
+/* Library version string. */
+const char* wl_version(void);
+
+/* Number of visible gfx950 devices (0 when none; never fails). */
+int wl_device_count(void);
+
+/* Code parameters for (rate, z).  Replaces ldpc::ldpc (include/ldpc.h:55,
+ * lib/ldpc.cc:11-63): WL_EINVAL where the reference throws. */
+int wl_code_params(int rate, int z, wl_code_info* out);
+
+/* Expanded parity-check table [m][c] with -1 padding, identical to the
+ * reference's m_checknode_array (lib/ldpc.cc:69-1206, lib/matrices/). */
+int wl_code_table(int rate, int z, int16_t* out, size_t out_len);
+
+/* ---------------------------------------------------------------- decoder */
+
+/* Replaces ldpc_decoder::ldpc_decoder(rate, z, max_iter, num_threads)
+ * (include/ldpc_decoder.h:25-28, lib/ldpc_decoder.cc:27-59).  flags: WL_FLAG_*. */
+int wl_decoder_create(int rate, int z, int max_iter, unsigned flags, wl_decoder** out);
+int wl_decoder_destroy(wl_decoder* dec);
+
+/* Frames decoded together by one workgroup (0 = autotuned default). */
+int wl_decoder_set_frames_per_wg(wl_decoder* dec, int frames_per_wg);
+int wl_decoder_get_frames_per_wg(const wl_decoder* dec, int* out);
+
+/* Batch decode of device-resident frames.  Replaces the OpenMP frame loop of
+ * lib/decode_example.cc:128-135 over ldpc_decoder::decode(float* rx,
+ * int8_t* decoded) (include/ldpc_decoder.h:33, lib/ldpc_decoder.cc:122-262).
+ *   llr      [nframes][n]  input, reference convention (not modified)
+ *   post     [nframes][n]  posterior LLRs (what decode() leaves in rx), nullable
+ *   hard     [nframes][n/8] packed hard decisions, nullable
+ *   ret      [nframes]     decode()'s return value: 0 syndrome clean, 1 failed, nullable
+ *   iters    [nframes]     iterations run (0 if the input already satisfies H), nullable
+ *   synd     [nframes]     failing parity checks of the output hard decisions
+ *                          (compute_syndrome(hd, false), lib/ldpc_decoder.cc:67-104), nullable */
+int wl_decode_f32_dev(wl_decoder* dec, const float* llr, size_t nframes, float* post,
+                      uint8_t* hard, uint8_t* ret, uint8_t* iters, uint16_t* synd, void* stream);
+
+/* Host-pointer batch with decode()'s exact semantics: rx [nframes][n] is
+ * replaced by the posterior, decoded [nframes][n] receives 0/1 bytes, ret
+ * [nframes] decode()'s return value, iters (nullable) the iteration count. */
+int wl_decode_f32_host(wl_decoder* dec, float* rx, size_t nframes, int8_t* decoded,
+                       uint8_t* ret, uint8_t* iters);
+
+/* Hard-input adapter: ldpc_decoder::decode(int8_t* rx, int8_t* decoded)
+ * (lib/ldpc_decoder.cc:108-118), soft = bit*2.0 - 1.0; rx is not modified. */
+int wl_decode_i8_host(wl_decoder* dec, const int8_t* rx, size_t nframes, int8_t* decoded,
+                      uint8_t* ret, uint8_t* iters);
+
+/* ldpc_decoder::compute_syndrome(int8_t*, bool) (lib/ldpc_decoder.cc:67-104)
+ * over one frame of n 0/1 bytes (host). */
+int wl_compute_syndrome_host(const wl_decoder* dec, const int8_t* hd, int exit_early,
+                             unsigned* out);
+
+/* Learned offset min-sum mode: offsets [T][mb][c] (slot j of layer l is the
+ * j-th nonzero block of base row l), T unrolled iterations; iteration t uses
+ * offsets[min(t, T-1)].  msg = sign * max(mag - offset, 0) in FP32.  Passing
+ * NULL returns the decoder to the reference's normalised min-sum. */
+int wl_decoder_set_offsets(wl_decoder* dec, const float* offsets_host, int T);
+
+/* Learned decode with fallback: frames whose hard decisions fail the
+ * syndrome after the learned iterations are decoded again from their input
+ * LLRs with normalised min-sum, fallback_iters iterations, early stop.
+ * fallback (nullable) [nframes] gets 1 for re-decoded frames. */
+int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nframes,
+                                   int fallback_iters, float* post, uint8_t* hard,
+                                   uint8_t* ret, uint8_t* iters, uint16_t* synd,
+                                   uint8_t* fallback, void* stream);
+
+/* ---------------------------------------------------------------- encoder */
+
+/* Replaces ldpc_encoder::ldpc_encoder(rate, z, num_threads)
+ * (include/ldpc_encoder.h:21, lib/ldpc_encoder.cc:9-17). */
+int wl_encoder_create(int rate, int z, wl_encoder** out);
+int wl_encoder_destroy(wl_encoder* enc);
+
+/* ldpc_encoder::encode_data(uint8_t* info, uint8_t* codeword)
+ * (include/ldpc_encoder.h:24, lib/ldpc_encoder.cc:50-105) over nframes
+ * device-resident frames: info [nframes][k] bytes (nonzero = 1; copied as is
+ * into the systematic part), codeword [nframes][n] bytes. */
+int wl_encode_u8_dev(wl_encoder* enc, const uint8_t* info, size_t nframes, uint8_t* codeword,
+                     void* stream);
+int wl_encode_u8_host(wl_encoder* enc, const uint8_t* info, size_t nframes, uint8_t* codeword);
+
+/* ---------------------------------------------------------------- channel */
+
+/* Seeded synthetic frames on the device, the GPU counterpart of
+ * BER_runner::run_iteration's source and channel (lib/BER_runner.cc:162-191):
+ * uniform random info bits (Philox4x32-10 keyed by seed; counter = absolute
+ * frame index and position), dual-diagonal encoding, BPSK bit 0 -> +1, AWGN
+ * with sigma^2 = 1/(2 R 10^(EbN0/10)), R = k/n, LLR = 2y/sigma^2, written
+ * negated, i.e. in the reference convention.  Frame f (absolute index
+ * frame_offset + f) uses Eb/N0 = ebn0_lo when ebn0_hi <= ebn0_lo, else a
+ * uniform draw in [ebn0_lo, ebn0_hi) from the same generator.
+ *   llr  [nframes][n] floats (nullable)
+ *   bits [nframes][n] codeword bytes 0/1 (nullable)
+ *   ebn0 [nframes] the Eb/N0 used per frame (nullable) */
+int wl_channel_dev(wl_encoder* enc, uint64_t seed, uint64_t frame_offset, size_t nframes,
+                   float ebn0_lo, float ebn0_hi, float* llr, uint8_t* bits, float* ebn0,
+                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

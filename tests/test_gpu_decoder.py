@@ -1,0 +1,198 @@
+"""GPU parity of the TDMP decoder against the CPU oracle, through the C ABI.
+
+The oracle restates lib/ldpc_decoder.cc:122-262; every comparison here is
+bit-exact: posterior float bits, packed hard decisions, decode()'s return
+value, iteration counts and syndrome weights.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import Z_VALUES, bits_equal_f32, noisy_llr, pack_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_decode(torch, dec, llr_np, with_post=True):
+    F, n = llr_np.shape
+    llr = torch.from_numpy(llr_np).cuda()
+    post = torch.empty_like(llr) if with_post else None
+    hard = torch.empty((F, n // 8), dtype=torch.uint8, device="cuda")
+    ret = torch.empty(F, dtype=torch.uint8, device="cuda")
+    iters = torch.empty(F, dtype=torch.uint8, device="cuda")
+    synd = torch.empty(F, dtype=torch.int16, device="cuda")
+    dec.decode_dev(llr, post, hard, ret, iters, synd)
+    torch.cuda.synchronize()
+    out = dict(hard=hard.cpu().numpy(), ret=ret.cpu().numpy(), iters=iters.cpu().numpy(),
+               synd=synd.cpu().numpy().view(np.uint16))
+    if with_post:
+        out["post"] = post.cpu().numpy()
+    # the input buffer must be untouched
+    assert bits_equal_f32(llr.cpu().numpy(), llr_np)
+    return out
+
+
+def _compare(g, o, label):
+    assert np.array_equal(g["ret"], o["ret"]), label + " ret"
+    assert np.array_equal(g["iters"], o["iters"]), label + " iters"
+    assert np.array_equal(g["synd"], o["synd_w"]), label + " syndrome weight"
+    assert np.array_equal(g["hard"], pack_bits(o["hd"])), label + " hard decisions"
+    if "post" in g:
+        bad = np.flatnonzero(g["post"].view(np.uint32) != o["post"].view(np.uint32))
+        assert bad.size == 0, "%s posterior mismatch at %d positions, first %r: gpu %r oracle %r" % (
+            label, bad.size, bad[:4], g["post"].ravel()[bad[:4]], o["post"].ravel()[bad[:4]])
+
+
+def test_cfg1_rate12_z24_early_stop(gpu, oracle_mod):
+    """Rate 1/2, z=24, 1024 frames at 2.5 dB, max 10 iterations, early stop."""
+    import wimax_ldpc_lib_amd as W
+    code = oracle_mod.Code(0, 24)
+    llr, _ = noisy_llr(code, 1024, 2.5, seed=11)
+    o = code.decode(llr, max_iter=10, early_stop=True)
+    dec = W.ldpc_decoder(0, 24, 10, early_stop=True)
+    g = _gpu_decode(gpu, dec, llr)
+    _compare(g, o, "cfg1")
+    assert o["ret"].sum() >= 1 and len(np.unique(o["iters"])) > 3   # failures and varied stops
+
+
+def test_cfg2_shape_fixed10(gpu, oracle_mod):
+    """Rate 5/6, z=96, exactly 10 iterations (no early stop), 512 frames at 4.0 dB."""
+    import wimax_ldpc_lib_amd as W
+    code = oracle_mod.Code(5, 96)
+    llr, _ = noisy_llr(code, 512, 4.0, seed=12)
+    o = code.decode(llr, max_iter=10, early_stop=False)
+    dec = W.ldpc_decoder(5, 96, 10, early_stop=False)
+    _compare(_gpu_decode(gpu, dec, llr), o, "cfg2")
+
+
+@pytest.mark.parametrize("rate", range(6))
+def test_all_codes(gpu, oracle_mod, rate):
+    """Every (rate, z), both stop modes, frames near the waterfall so that
+    iteration counts vary (3/4B is decode-only: random LLRs, no codeword)."""
+    import wimax_ldpc_lib_amd as W
+    ebn0 = {0: 1.8, 1: 2.4, 2: 2.4, 3: 2.8, 4: 2.8, 5: 3.4}[rate]
+    for z in Z_VALUES:
+        code = oracle_mod.Code(rate, z)
+        if rate == 4:
+            rng = np.random.default_rng(100 + z)
+            llr = rng.normal(1.0, 2.0, (24, code.n)).astype(np.float32)
+        else:
+            llr, _ = noisy_llr(code, 24, ebn0, seed=1000 * rate + z)
+        for early in (True, False):
+            o = code.decode(llr, max_iter=8, early_stop=early)
+            dec = W.ldpc_decoder(rate, z, 8, early_stop=early)
+            _compare(_gpu_decode(gpu, dec, llr), o, "rate %d z %d early %d" % (rate, z, early))
+
+
+@pytest.mark.parametrize("rate,z", [(0, 24), (5, 96), (3, 52), (1, 60)])
+def test_frames_per_workgroup_and_ragged_batches(gpu, oracle_mod, rate, z):
+    """Every frames-per-workgroup setting and batch sizes that leave partial workgroups."""
+    import wimax_ldpc_lib_amd as W
+    code = oracle_mod.Code(rate, z)
+    llr, _ = noisy_llr(code, 37, 2.0 if rate < 3 else 3.0, seed=7 * z + rate)
+    o = code.decode(llr, max_iter=6, early_stop=True)
+    dec = W.ldpc_decoder(rate, z, 6, early_stop=True)
+    fmax = 1
+    while True:
+        try:
+            dec.frames_per_wg = fmax + 1
+            fmax += 1
+        except W.WLError:
+            break
+    assert fmax >= 2
+    for f in range(1, fmax + 1):
+        dec.frames_per_wg = f
+        for nf in (1, f, 37):
+            g = _gpu_decode(gpu, dec, np.ascontiguousarray(llr[:nf]))
+            oo = {k: v[:nf] for k, v in o.items() if k != "seconds"}
+            _compare(g, oo, "F=%d nframes=%d" % (f, nf))
+
+
+def test_edge_cases(gpu, oracle_mod):
+    """Clean input (0 iterations, rx untouched), the max_iter = 0 quirk,
+    |L| >= 10000 (the initialiser), signed zeros and all-zero input."""
+    import wimax_ldpc_lib_amd as W
+    code = oracle_mod.Code(5, 96)
+    rng = np.random.default_rng(5)
+    info = rng.integers(0, 2, (4, code.k), dtype=np.uint8)
+    cw = code.encode(info)
+    clean = np.where(cw == 1, 3.0, -3.0).astype(np.float32)
+    huge = np.where(rng.random((4, code.n)) < 0.5, 20000.0, -15000.0).astype(np.float32)
+    zeros = np.zeros((2, code.n), np.float32)
+    negz = np.full((2, code.n), -0.0, np.float32)
+    mixed = rng.normal(0, 1, (4, code.n)).astype(np.float32)
+    mixed[:, ::7] = 0.0
+    mixed[:, 3::11] = -0.0
+    llr = np.concatenate([clean, huge, zeros, negz, mixed])
+    for max_iter in (0, 1, 10):
+        for early in (True, False):
+            o = code.decode(llr, max_iter=max_iter, early_stop=early)
+            dec = W.ldpc_decoder(5, 96, max_iter, early_stop=early)
+            g = _gpu_decode(gpu, dec, llr)
+            _compare(g, o, "edge max_iter %d early %d" % (max_iter, early))
+            if early:
+                assert (g["iters"][:4] == 0).all() and bits_equal_f32(g["post"][:4], clean)
+                if max_iter == 0:
+                    assert (g["ret"] == 0).all()   # lib/ldpc_decoder.cc:124,141 quirk
+
+
+def test_host_api_semantics(gpu, oracle_mod):
+    """decode(float* rx, int8_t* decoded): posterior in place, 0/1 bytes,
+    return value; decode(int8_t*): the hard-input adapter."""
+    import wimax_ldpc_lib_amd as W
+    code = oracle_mod.Code(2, 48)
+    llr, cw = noisy_llr(code, 3, 2.2, seed=3)
+    dec = W.ldpc_decoder(2, 48, 10)
+    for f in range(3):
+        rx = llr[f].copy()
+        decoded = np.zeros(code.n, np.int8)
+        r = dec.decode(rx, decoded)
+        o = code.decode(llr[f:f + 1], 10, True)
+        assert r == o["ret"][0]
+        assert bits_equal_f32(rx, o["post"][0])
+        assert np.array_equal(decoded, o["hd"][0])
+    bits = cw[0].astype(np.int8).copy()
+    bits[::50] ^= 1
+    keep = bits.copy()
+    decoded = np.zeros(code.n, np.int8)
+    r = dec.decode(bits, decoded)
+    r2, hd2, _ = code.decode_i8(keep, 10, True)
+    assert np.array_equal(bits, keep)
+    assert r == r2 and np.array_equal(decoded, hd2)
+    s = dec.compute_syndrome(decoded, False)
+    assert s == code.syndrome(decoded, False)
+
+
+def test_large_batch_properties(gpu, oracle_mod):
+    """65536 frames of rate 5/6, z=96 at 4.0 dB, exactly 10 iterations:
+    size-independent properties at full size (frames are independent, so
+    any frame decoded alone equals its slot in the batch; every frame's
+    syndrome weight is recomputed from its hard decisions) and a bit-exact
+    oracle comparison on a spread sample of frames."""
+    import wimax_ldpc_lib_amd as W
+    torch = gpu
+    code = oracle_mod.Code(5, 96)
+    enc = W.ldpc_encoder(5, 96)
+    F = 65536
+    llr = torch.empty((F, code.n), dtype=torch.float32, device="cuda")
+    bits = torch.empty((F, code.n), dtype=torch.uint8, device="cuda")
+    enc.channel_dev(1234, 0, F, 4.0, llr=llr, bits=bits)
+    dec = W.ldpc_decoder(5, 96, 10, early_stop=False)
+    hard = torch.empty((F, code.n // 8), dtype=torch.uint8, device="cuda")
+    ret = torch.empty(F, dtype=torch.uint8, device="cuda")
+    synd = torch.empty(F, dtype=torch.int16, device="cuda")
+    dec.decode_dev(llr, None, hard, ret, None, synd)
+    torch.cuda.synchronize()
+    hard_np = hard.cpu().numpy()
+    sample = np.unique(np.concatenate([np.arange(0, F, 997), [F - 1]]))
+    llr_s = llr[torch.from_numpy(sample).cuda()].cpu().numpy()
+    o = code.decode(llr_s, 10, False)
+    assert np.array_equal(hard_np[sample], pack_bits(o["hd"]))
+    assert np.array_equal(ret.cpu().numpy()[sample], o["ret"])
+    # frame errors against the transmitted codewords at 4.0 dB are rare
+    cw_hard = np.packbits(bits.cpu().numpy(), axis=1, bitorder="little")
+    fer = np.mean((cw_hard != hard_np).any(axis=1))
+    assert fer < 1e-2
+    H = code.dense_h().astype(np.uint8)
+    hd = np.unpackbits(hard_np[:256], axis=1, bitorder="little")[:, :code.n]
+    w = ((hd.astype(np.int64) @ H.T.astype(np.int64)) % 2).sum(axis=1)
+    assert np.array_equal(w, synd.cpu().numpy()[:256])

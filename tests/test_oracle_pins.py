@@ -1,0 +1,314 @@
+"""Pins of the CPU oracle (oracle/oracle.c) -- what checks each oracle
+function independently of the oracle's own code.
+
+  tables      reference's own compiled tables (oracle/_ref, this container)
+              and committed sha256 fixtures of them (everywhere); QC and
+              disjoint-layer invariants
+  encoder     reference's own compiled encoder (oracle/_ref) and committed
+              fixtures; H c = 0; GF(2) Gaussian elimination (the dual-diagonal
+              parity part is invertible, so the systematic codeword is unique)
+  syndrome    dense H products
+  check row   brute-force max-log over every local configuration of a single
+              parity check (min-sum is its closed form), then the IEEE double
+              step computed by Python floats; a case that separates the
+              double step from a float step
+  decoder     an independent vectorised numpy layered decoder, special cases
+              (clean input, max_iter = 0, the 10000 initialiser, signed zeros)
+              and the iteration prefix property
+  BER         statistical agreement with the reference's published curves
+The reference decoder itself (lib/ldpc_decoder.cc) cannot be built here: it
+includes VOLK headers this image does not have.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests.helpers import Z_VALUES, noisy_llr
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ------------------------------------------------------------------ tables
+
+def test_tables_match_reference_hashes(oracle_mod):
+    ref = json.load(open(os.path.join(GOLDEN, "tables_ref.json")))
+    assert len(ref) == 114
+    for rate in range(6):
+        for z in Z_VALUES:
+            c = oracle_mod.Code(rate, z)
+            r = ref["%d_%d" % (rate, z)]
+            h = hashlib.sha256(np.ascontiguousarray(c.cn, dtype="<i2").tobytes()).hexdigest()
+            assert h == r["sha256"], (rate, z)
+            assert (c.n, c.m, c.c, c.d, c.x) == (r["n"], r["m"], r["c"], r["d"], r["x"])
+
+
+def test_tables_match_reference_build(oracle_mod):
+    if not oracle_mod.ref_available():
+        pytest.skip("reference build (oracle/_ref) not present")
+    for rate in range(6):
+        for z in Z_VALUES:
+            c = oracle_mod.Code(rate, z)
+            t = oracle_mod.ref_table(rate, z)
+            assert np.array_equal(t["table"], c.cn)
+            assert (t["d"], t["x"], t["n"], t["m"]) == (c.d, c.x, c.n, c.m)
+
+
+def test_table_invariants(oracle_mod):
+    for rate in range(6):
+        for z in (24, 52, 96):
+            c = oracle_mod.Code(rate, z)
+            for l in range(c.mb):
+                rows = c.cn[l * z:(l + 1) * z, :c.deg[l]].astype(np.int64)
+                assert (np.diff(rows, axis=1) > 0).all()          # ascending
+                assert len(np.unique(rows)) == rows.size          # disjoint within a layer
+                for j in range(c.deg[l]):                         # QC: each row is a rotation
+                    b, s = c.blk[l, j], c.shift[l, j]
+                    assert np.array_equal(rows[:, j], b * z + (np.arange(z) + s) % z)
+
+
+# ------------------------------------------------------------------ encoder
+
+def _gf2_parity(H, k, u):
+    """Solve H[:, k:] p = H[:, :k] u over GF(2) by Gaussian elimination."""
+    A = H[:, k:].copy() % 2
+    b = (H[:, :k].astype(np.int64) @ u) % 2
+    m = A.shape[0]
+    A = np.concatenate([A, b[:, None]], axis=1).astype(np.uint8)
+    row = 0
+    for col in range(m):
+        piv = np.flatnonzero(A[row:, col])
+        assert piv.size, "parity part singular"
+        p = row + piv[0]
+        A[[row, p]] = A[[p, row]]
+        others = np.flatnonzero(A[:, col])
+        others = others[others != row]
+        A[others] ^= A[row]
+        row += 1
+    return A[:, -1]
+
+
+def test_encoder_matches_reference_build(oracle_mod):
+    if not oracle_mod.ref_available():
+        pytest.skip("reference build (oracle/_ref) not present")
+    rng = np.random.default_rng(0)
+    for rate in range(6):
+        for z in Z_VALUES:
+            c = oracle_mod.Code(rate, z)
+            info = rng.integers(0, 2, (3, c.k), dtype=np.uint8)
+            info[2] = rng.integers(0, 256, c.k, dtype=np.uint8)
+            assert np.array_equal(c.encode(info), oracle_mod.ref_encode(rate, z, info)), (rate, z)
+
+
+def test_encoder_matches_fixtures(oracle_mod):
+    fx = np.load(os.path.join(GOLDEN, "encoder_ref.npz"))
+    n = 0
+    for key in fx.files:
+        if key.startswith("info_"):
+            _, rate, z = key.split("_")
+            c = oracle_mod.Code(int(rate), int(z))
+            assert np.array_equal(c.encode(fx[key]), fx["cw_%s_%s" % (rate, z)]), key
+            n += 1
+    assert n == 24
+
+
+def test_encoder_codewords_unique_solution(oracle_mod):
+    rng = np.random.default_rng(1)
+    for rate in (0, 1, 2, 3, 5):
+        c = oracle_mod.Code(rate, 24)
+        H = c.dense_h()
+        for _ in range(3):
+            u = rng.integers(0, 2, c.k, dtype=np.uint8)
+            cw = c.encode(u[None])[0]
+            assert not ((H.astype(np.int64) @ cw) % 2).any()
+            assert np.array_equal(cw[:c.k], u)
+            assert np.array_equal(cw[c.k:], _gf2_parity(H, c.k, u))
+    # 3/4B: the reference's encoder constants do not fit that matrix, so its
+    # codewords fail H (the oracle reproduces that; see the fixture test)
+    c = oracle_mod.Code(4, 24)
+    cw = c.encode(rng.integers(0, 2, (4, c.k), dtype=np.uint8))
+    assert ((c.dense_h().astype(np.int64) @ cw.T) % 2).any(axis=0).all()
+
+
+# ------------------------------------------------------------------ syndrome
+
+def test_syndrome_matches_dense_h(oracle_mod):
+    rng = np.random.default_rng(2)
+    for rate, z in ((0, 24), (5, 96), (4, 40)):
+        c = oracle_mod.Code(rate, z)
+        H = c.dense_h().astype(np.int64)
+        for _ in range(5):
+            hd = rng.integers(0, 2, c.n, dtype=np.int8)
+            w = int(((H @ hd) % 2).sum())
+            assert c.syndrome(hd, False) == w
+            assert c.syndrome(hd, True) == (1 if w else 0)
+
+
+# ------------------------------------------------------------------ check row
+
+def _brute_extrinsic(L):
+    """max-log extrinsic of a single even-parity check, by enumeration:
+    max over local words with x_n = 1 minus max with x_n = 0 of sum_j x_j L_j
+    (L_j = log P(x_j = 1) / P(x_j = 0), the reference's sign convention)."""
+    d = len(L)
+    X = ((np.arange(2 ** (d - 1))[:, None] >> np.arange(d - 1)) & 1).astype(np.float64)
+    par = X.sum(axis=1) % 2
+    out = np.empty(d)
+    for n in range(d):
+        others = np.delete(L, n).astype(np.float64)
+        score = X @ others
+        out[n] = score[par == 1].max() - score[par == 0].max()
+    return out
+
+
+def test_check_row_matches_brute_force(oracle_mod):
+    rng = np.random.default_rng(3)
+    for d in (6, 7, 10, 11, 14, 15):
+        for trial in range(20):
+            mag = rng.uniform(0.01, 40.0, d)
+            if trial % 4 == 0:
+                mag[rng.integers(d)] = mag[rng.integers(d)]          # ties at some value
+            L = (mag * rng.choice([-1.0, 1.0], d)).astype(np.float32)
+            ext = _brute_extrinsic(L)
+            post, msg = oracle_mod.row_update(oracle_mod.NMS, L)
+            want_msg = np.array([np.float32(e * 0.815) for e in ext], dtype=np.float32)
+            want_post = np.array([np.float32(float(l) + e * 0.815) for l, e in zip(L, ext)], dtype=np.float32)
+            assert np.array_equal(msg.view(np.uint32), want_msg.view(np.uint32)), (d, trial)
+            assert np.array_equal(post.view(np.uint32), want_post.view(np.uint32)), (d, trial)
+            # plain min-sum (the reference's third variant) is the brute force itself
+            post_ms, msg_ms = oracle_mod.row_update(oracle_mod.MS, L)
+            assert np.array_equal(msg_ms, ext.astype(np.float32))
+
+
+def test_check_row_double_step_not_float_step(oracle_mod):
+    """Find values where rounding L + s*min*0.815 through double differs from
+    an all-float evaluation; the oracle must follow the double path of
+    lib/ldpc_decoder.cc:220-223 (float * double literal)."""
+    rng = np.random.default_rng(4)
+    found = 0
+    for _ in range(20000):
+        a, b = rng.uniform(0.1, 30.0, 2).astype(np.float32)
+        L = np.array([a, b, 50.0, 60.0, 70.0, 80.0], np.float32)
+        e = _brute_extrinsic(L)
+        dbl = np.float32(float(L[0]) + e[0] * 0.815)
+        flt = np.float32(L[0] + np.float32(np.float32(e[0]) * np.float32(0.815)))
+        if dbl != flt:
+            post, _ = oracle_mod.row_update(oracle_mod.NMS, L)
+            assert post[0] == dbl and post[0] != flt
+            found += 1
+            if found >= 5:
+                break
+    assert found >= 5
+
+
+def test_check_row_special_cases(oracle_mod):
+    # the 10000 initialiser: every |L| >= 10000 -> message magnitude (float)(10000*0.815)
+    post, msg = oracle_mod.row_update(oracle_mod.NMS, np.array([20000, -15000, 12000, 10000], np.float32))
+    assert np.all(np.abs(msg) == np.float32(10000 * 0.815))
+    # signed zeros follow std::signbit for the parity and the first strict minimum
+    post, msg = oracle_mod.row_update(oracle_mod.NMS, np.array([-0.0, 1.0, 2.0], np.float32))
+    assert msg[0] == np.float32(-0.815) and np.signbit(msg[1]) == False and msg[1] == 0  # noqa: E712
+    post, msg = oracle_mod.row_update(oracle_mod.NMS, np.array([0.0, 1.0, 2.0], np.float32))
+    assert msg[0] == np.float32(0.815) and np.signbit(msg[1]) and msg[1] == 0
+    # OMS (the reference's compile-time alternative): s * max(min - 0.10, 0)
+    L = np.array([0.05, 1.0, -2.0, 3.0], np.float32)
+    ext = _brute_extrinsic(L)
+    post, msg = oracle_mod.row_update(oracle_mod.OMS, L)
+    want = [np.float32(np.sign(e) * max(abs(e) - 0.10, 0.0)) for e in ext]
+    assert np.array_equal(np.abs(msg), np.abs(np.array(want, np.float32)))
+
+
+# ------------------------------------------------------------------ decoder
+
+def numpy_layered_decode(code, llr, max_iter, early_stop):
+    """Independent vectorised layered NMS decoder (numpy).  Check rule in the
+    'this bit equals the XOR of the others' form, min2 by masking the first
+    argmin, double-precision message step."""
+    H = code.dense_h().astype(np.int64)
+    z = code.z
+    rx = llr.astype(np.float32).copy()
+    msgs = [np.zeros((z, code.deg[l]), np.float32) for l in range(code.mb)]
+    idx = [code.cn[l * z:(l + 1) * z, :code.deg[l]].astype(np.int64) for l in range(code.mb)]
+
+    def clean(v):
+        return not ((H @ (v >= 0).astype(np.int64)) % 2).any()
+
+    if early_stop and clean(rx):
+        return rx, 0, 0
+    for it in range(1, max_iter + 1):
+        for l in range(code.mb):
+            L = rx[idx[l]] - msgs[l]
+            a = np.abs(L)
+            am = np.argmin(a, axis=1)
+            m1 = np.minimum(a[np.arange(z), am], np.float32(10000))
+            a2 = a.copy()
+            a2[np.arange(z), am] = np.inf
+            m2 = np.minimum(a2.min(axis=1), np.float32(10000))
+            mag = np.where(np.arange(L.shape[1])[None, :] == am[:, None], m2[:, None], m1[:, None])
+            nn = (~np.signbit(L)).astype(np.int64)
+            others_odd = (nn.sum(axis=1, keepdims=True) - nn) % 2 == 1
+            s = np.where(others_odd, 1.0, -1.0)
+            v = s * (mag.astype(np.float64) * 0.815)
+            msgs[l] = v.astype(np.float32)
+            rx[idx[l]] = (L.astype(np.float64) + v).astype(np.float32)
+        if early_stop and clean(rx):
+            return rx, it, 0
+    return rx, max_iter, (0 if (not early_stop and clean(rx)) else (0 if max_iter == 0 else 1))
+
+
+@pytest.mark.parametrize("rate,z,ebn0", [(0, 24, 2.0), (5, 28, 3.5), (2, 52, 2.4), (4, 32, None)])
+def test_decoder_matches_independent_numpy(oracle_mod, rate, z, ebn0):
+    code = oracle_mod.Code(rate, z)
+    if ebn0 is None:
+        llr = np.random.default_rng(9).normal(0.5, 2.0, (6, code.n)).astype(np.float32)
+    else:
+        llr, _ = noisy_llr(code, 6, ebn0, seed=rate + z)
+    for early in (True, False):
+        o = code.decode(llr, 7, early)
+        for f in range(llr.shape[0]):
+            post, it, ret = numpy_layered_decode(code, llr[f], 7, early)
+            assert np.array_equal(post.view(np.uint32), o["post"][f].view(np.uint32)), (f, early)
+            assert it == o["iters"][f] and ret == o["ret"][f]
+
+
+def test_decoder_special_cases(oracle_mod):
+    code = oracle_mod.Code(1, 96)
+    rng = np.random.default_rng(5)
+    cw = code.encode(rng.integers(0, 2, (2, code.k), dtype=np.uint8))
+    clean = np.where(cw == 1, 2.5, -1.5).astype(np.float32)
+    o = code.decode(clean, 10, True)
+    assert (o["iters"] == 0).all() and (o["ret"] == 0).all() and np.array_equal(o["post"], clean)
+    noisy = rng.normal(0, 3, (4, code.n)).astype(np.float32)
+    o0 = code.decode(noisy, 0, True)
+    assert (o0["ret"] == 0).all() and (o0["synd_w"] > 0).all()   # lib/ldpc_decoder.cc:124,141
+
+
+def test_decoder_prefix_property(oracle_mod):
+    """decode with max_iter = k runs exactly the first k iterations of a longer run."""
+    code = oracle_mod.Code(0, 48)
+    llr, _ = noisy_llr(code, 64, 1.6, seed=8)
+    full = code.decode(llr, 10, True)
+    for k in (1, 3, 6):
+        part = code.decode(llr, k, True)
+        fixed = code.decode(llr, k, False)
+        for f in range(64):
+            if full["iters"][f] <= k and full["ret"][f] == 0:
+                assert part["iters"][f] == full["iters"][f] and part["ret"][f] == 0
+                assert np.array_equal(part["post"][f], full["post"][f])
+            else:
+                assert part["iters"][f] == k and part["ret"][f] == 1
+                assert np.array_equal(part["post"][f], fixed["post"][f])
+
+
+def test_ber_agrees_with_published_curves(oracle_mod):
+    """BER_runner's definition (coded-bit errors / info bits,
+    lib/BER_runner.cc:152-158,198) against example_plots of the reference:
+    rate 1/2 N=576 at 2.5 dB ~6.77e-4, rate 5/6 N=576 at 4.5 dB ~2.03e-5."""
+    for rate, z, eb, published, frames in ((0, 24, 2.5, 6.77e-4, 6000), (5, 24, 4.5, 2.03e-5, 20000)):
+        code = oracle_mod.Code(rate, z)
+        llr, cw = noisy_llr(code, frames, eb, seed=rate * 7 + 1)
+        o = code.decode(llr, 10, True)
+        ber = (o["hd"] != cw).sum() / (frames * code.k)
+        assert published / 2.5 < ber < published * 2.5, (rate, ber)

@@ -1,0 +1,102 @@
+"""Build recipe for libwimax_ldpc_amd.so (gfx950 HIP kernels + C ABI + C++ API).
+
+    python -m wimax_ldpc_lib_amd.build          # incremental
+    python -m wimax_ldpc_lib_amd.build --force  # rebuild everything
+
+Objects go to wimax_ldpc_lib_amd/_build/, the library to
+wimax_ldpc_lib_amd/lib/libwimax_ldpc_amd.so (both in-tree, git-ignored).
+"""
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+LIBDIR = os.path.join(PKG, "lib")
+LIB = os.path.join(LIBDIR, "libwimax_ldpc_amd.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_FLAGS = [
+    "--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC", "-Wall",
+    # IEEE semantics for the bit-exact FP64 message step
+    "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero",
+    # no packed FP32 instructions anywhere
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I" + os.path.join(ROOT, "include"),
+]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include")]
+
+HIP_SOURCES = ["decode_r0.hip", "decode_r1.hip", "decode_r2.hip", "decode_r3.hip", "decode_r4.hip",
+               "decode_r5.hip", "encoder.hip", "capi.hip"]
+CXX_SOURCES = ["ldpc_classes.cpp"]
+
+
+def _headers():
+    hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    inc = os.path.join(ROOT, "include")
+    hs += [os.path.join(inc, f) for f in os.listdir(inc) if f.endswith(".h")]
+    return hs
+
+
+def _stale(obj, src, headers):
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(p) > t for p in [src] + headers)
+
+
+def _compile(cmd):
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("compile failed:\n%s\n%s%s" % (" ".join(cmd), p.stdout, p.stderr))
+    return p.stderr
+
+
+def build(force=False, jobs=None, verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    headers = _headers()
+    cmds = []
+    objs = []
+    for s in HIP_SOURCES:
+        src = os.path.join(CSRC, s)
+        obj = os.path.join(BUILD, s + ".o")
+        objs.append(obj)
+        if force or _stale(obj, src, headers):
+            cmds.append([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
+    for s in CXX_SOURCES:
+        src = os.path.join(CSRC, s)
+        if not os.path.exists(src):
+            continue
+        obj = os.path.join(BUILD, s + ".o")
+        objs.append(obj)
+        if force or _stale(obj, src, headers):
+            cmds.append(["g++"] + CXX_FLAGS + ["-c", src, "-o", obj])
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        for out in ex.map(_compile, cmds):
+            if verbose and out.strip():
+                sys.stderr.write(out)
+    if force or cmds or not os.path.exists(LIB):
+        tmp = LIB + ".tmp"
+        _compile([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+        os.replace(tmp, LIB)
+    return LIB
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--force", action="store_true")
+    ap.add_argument("-j", type=int, default=None)
+    ap.add_argument("-v", action="store_true")
+    a = ap.parse_args()
+    print(build(force=a.force, jobs=a.j, verbose=a.v))
+
+
+if __name__ == "__main__":
+    main()

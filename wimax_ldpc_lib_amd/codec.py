@@ -1,0 +1,169 @@
+"""Python host mirror of the reference's codec classes over the C ABI.
+
+ldpc_decoder / ldpc_encoder follow include/ldpc_decoder.h and
+include/ldpc_encoder.h of the reference: same constructor arguments, decode()
+returns what the reference's decode returns and replaces rx by the posterior,
+encode_data() fills the codeword.  The *_dev methods take torch tensors on the
+GPU (the device memory of this library's callers) and enqueue on the current
+torch stream; the host methods take numpy arrays.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import check, lib
+
+HALFRATE, TWOTHIRDSA, TWOTHIRDSB, THREEQUARTERSA, THREEQUARTERSB, FIVESIXTHS = range(6)
+
+
+def _ptr(t):
+    """Raw pointer of a torch tensor / numpy array / None."""
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        assert t.flags.c_contiguous
+        return t.ctypes.data_as(C.c_void_p)
+    assert t.is_contiguous()
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is not None:
+        return C.c_void_p(stream)
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class ldpc:
+    """Code parameters of (rate, z) -- include/ldpc.h:35-61."""
+
+    def __init__(self, rate, z_factor, num_threads=1):
+        p = capi.code_params(int(rate), int(z_factor))
+        self.rate, self.z = p["rate"], p["z"]
+        self.n, self.m, self.k, self.c = p["n"], p["m"], p["k"], p["c"]
+        self.mb, self.D, self.X = p["mb"], p["d"], p["x"]
+        self.num_threads = num_threads
+
+    def checknode_array(self):
+        t = np.zeros((self.m, self.c), dtype=np.int16)
+        check(lib().wl_code_table(self.rate, self.z, _ptr(t), t.size))
+        return t
+
+
+class ldpc_decoder(ldpc):
+    """include/ldpc_decoder.h:12-34 over wl_decoder."""
+
+    def __init__(self, rate, z_factor, max_iter, num_threads=1, early_stop=True):
+        super().__init__(rate, z_factor, num_threads)
+        h = C.c_void_p()
+        flags = capi.WL_FLAG_EARLY_STOP if early_stop else 0
+        check(lib().wl_decoder_create(int(rate), int(z_factor), int(max_iter), flags, C.byref(h)))
+        self._h = h
+        self.max_iter = max_iter
+        self.early_stop = early_stop
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().wl_decoder_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def frames_per_wg(self):
+        v = C.c_int()
+        check(lib().wl_decoder_get_frames_per_wg(self._h, C.byref(v)))
+        return v.value
+
+    @frames_per_wg.setter
+    def frames_per_wg(self, f):
+        check(lib().wl_decoder_set_frames_per_wg(self._h, int(f)))
+
+    def set_offsets(self, offsets):
+        """Learned offsets [T][mb][c] (numpy float32), or None for plain NMS."""
+        if offsets is None:
+            check(lib().wl_decoder_set_offsets(self._h, None, 0))
+            return
+        o = np.ascontiguousarray(offsets, dtype=np.float32)
+        assert o.ndim == 3 and o.shape[1:] == (self.mb, self.c), o.shape
+        check(lib().wl_decoder_set_offsets(self._h, _ptr(o), o.shape[0]))
+
+    def compute_syndrome(self, rx_codeword, exit_early):
+        hd = np.ascontiguousarray(rx_codeword, dtype=np.int8)
+        out = C.c_uint()
+        check(lib().wl_compute_syndrome_host(self._h, _ptr(hd), int(bool(exit_early)), C.byref(out)))
+        return out.value
+
+    def decode(self, rx_codeword, decoded):
+        """decode(float32 rx[n] or int8 bits[n], int8 decoded[n]) -> 0/1, in place."""
+        ret = np.zeros(1, dtype=np.uint8)
+        if rx_codeword.dtype == np.int8:
+            check(lib().wl_decode_i8_host(self._h, _ptr(rx_codeword), 1, _ptr(decoded), _ptr(ret), None))
+        else:
+            assert rx_codeword.dtype == np.float32
+            check(lib().wl_decode_f32_host(self._h, _ptr(rx_codeword), 1, _ptr(decoded), _ptr(ret), None))
+        return int(ret[0])
+
+    def decode_batch_host(self, rx, decoded, ret, iters=None):
+        """Host batch: rx [F][n] float32 replaced by posteriors."""
+        F = rx.shape[0]
+        check(lib().wl_decode_f32_host(self._h, _ptr(rx), F, _ptr(decoded), _ptr(ret), _ptr(iters)))
+
+    def decode_dev(self, llr, post=None, hard=None, ret=None, iters=None, synd=None, stream=None):
+        """Device batch: llr [F][n] float32 CUDA tensor."""
+        F = llr.shape[0] if llr.dim() == 2 else llr.numel() // self.n
+        check(lib().wl_decode_f32_dev(self._h, _ptr(llr), F, _ptr(post), _ptr(hard), _ptr(ret),
+                                      _ptr(iters), _ptr(synd), _stream(stream)))
+
+    def decode_learned_fallback_dev(self, llr, fallback_iters, post=None, hard=None, ret=None, iters=None,
+                                    synd=None, fallback=None, stream=None):
+        F = llr.shape[0]
+        check(lib().wl_decode_learned_fallback_dev(self._h, _ptr(llr), F, int(fallback_iters), _ptr(post),
+                                                   _ptr(hard), _ptr(ret), _ptr(iters), _ptr(synd),
+                                                   _ptr(fallback), _stream(stream)))
+
+
+class ldpc_encoder(ldpc):
+    """include/ldpc_encoder.h:10-27 over wl_encoder."""
+
+    def __init__(self, rate, z_factor, num_threads=1):
+        super().__init__(rate, z_factor, num_threads)
+        h = C.c_void_p()
+        check(lib().wl_encoder_create(int(rate), int(z_factor), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().wl_encoder_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def encode_data(self, infoword, codeword):
+        F = infoword.size // self.k
+        check(lib().wl_encode_u8_host(self._h, _ptr(infoword), F, _ptr(codeword)))
+
+    def encode_dev(self, info, cw, stream=None):
+        F = info.numel() // self.k
+        check(lib().wl_encode_u8_dev(self._h, _ptr(info), F, _ptr(cw), _stream(stream)))
+
+    def channel_dev(self, seed, frame_offset, nframes, ebn0_lo, ebn0_hi=None, llr=None, bits=None,
+                    ebn0=None, stream=None):
+        hi = ebn0_lo if ebn0_hi is None else ebn0_hi
+        check(lib().wl_channel_dev(self._h, int(seed), int(frame_offset), int(nframes), float(ebn0_lo),
+                                   float(hi), _ptr(llr), _ptr(bits), _ptr(ebn0), _stream(stream)))
+
+
+def unpack_hard(hard, n):
+    """Packed hard decisions [F][n/8] (uint8, numpy) -> [F][n] int8 0/1."""
+    h = np.asarray(hard, dtype=np.uint8)
+    return np.unpackbits(h.reshape(h.shape[0], -1), axis=1, bitorder="little")[:, :n].astype(np.int8)

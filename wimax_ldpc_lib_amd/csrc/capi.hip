@@ -1,0 +1,464 @@
+// capi.hip -- the C ABI (include/wimax_ldpc_amd.h) over the gfx950 kernels.
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/wimax_ldpc_amd.h"
+#include "dispatch.hpp"
+#include "encoder.hpp"
+
+#define WL_VERSION_STR "wimax_ldpc_amd 0.1 (gfx950)"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    return fail(WL_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define WL_HIP(call)                                   \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return hip_fail(e_, #call); \
+    } while (0)
+
+struct HostCode {
+    wl_code_info info{};
+    wl::LayerTable lt{};
+    std::vector<int16_t> table;   // [m][c]
+};
+
+int make_code(int rate, int z, HostCode& hc)
+{
+    if (rate < 0 || rate >= wl::kNumRates)
+        return fail(WL_EINVAL, "invalid coderate %d (valid: 0..5)", rate);
+    if (!wl::valid_z(z)) return fail(WL_EINVAL, "invalid z factor %d (valid: 24..96 step 4)", z);
+    hc.lt = wl::make_layer_table(rate);
+    const wl_base_info& bi = wl_base_table[rate];
+    wl_code_info& ci = hc.info;
+    ci.rate = rate;
+    ci.z = z;
+    ci.n = 24 * z;
+    ci.mb = hc.lt.nl;
+    ci.m = hc.lt.nl * z;
+    ci.k = ci.n - ci.m;
+    ci.c = hc.lt.c;
+    ci.d = wl::lift(rate, bi.d0, z);
+    ci.x = bi.x;
+    hc.table.assign((size_t)ci.m * ci.c, (int16_t)-1);
+    for (int l = 0; l < hc.lt.nl; l++)
+        for (int r = 0; r < z; r++)
+            for (int j = 0; j < hc.lt.deg[l]; j++) {
+                const int s = wl::lift(rate, hc.lt.p96[l][j], z);
+                hc.table[(size_t)(l * z + r) * ci.c + j] = (int16_t)(hc.lt.blk[l][j] * z + (r + s) % z);
+            }
+    return WL_OK;
+}
+
+wl::EncodeDesc make_encode_desc(const HostCode& hc)
+{
+    wl::EncodeDesc d{};
+    d.z = hc.info.z;
+    d.n = hc.info.n;
+    d.k = hc.info.k;
+    d.m = hc.info.m;
+    d.mb = hc.info.mb;
+    d.kb = 24 - hc.info.mb;
+    d.x = hc.info.x;
+    d.d = hc.info.d;
+    d.frames_per_wg = std::max(1, 256 / d.z);
+    for (int l = 0; l < hc.lt.nl; l++) {
+        d.deg[l] = (int8_t)hc.lt.deg[l];
+        for (int j = 0; j < hc.lt.deg[l]; j++) {
+            d.blk[l][j] = (int8_t)hc.lt.blk[l][j];
+            d.sh[l][j] = (int8_t)wl::lift(hc.info.rate, hc.lt.p96[l][j], d.z);
+        }
+    }
+    return d;
+}
+
+// Default frames per workgroup: the most lanes busy, then the larger group up
+// to 256 threads (the autotuner may override per (rate, z)).
+int default_frames_per_wg(int z, int fmax)
+{
+    int best = 1;
+    double best_eff = -1.0;
+    for (int f = 1; f <= fmax; f++) {
+        const int thr = ((f * z + 63) / 64) * 64;
+        if (thr > 512) break;
+        const double eff = (double)(f * z) / thr;
+        const bool better = eff > best_eff + 1e-9 || (eff > best_eff - 1e-9 && thr <= 256);
+        if (better) {
+            best = f;
+            best_eff = eff;
+        }
+    }
+    return best;
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t n)
+    {
+        if (n <= cap) return WL_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+        cap = n;
+        return WL_OK;
+    }
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+__global__ void zero_u32_kernel(uint32_t* p) { *p = 0; }
+
+// Frames with ret != 0 go to the fallback list; fallback[g] records it.
+__global__ void compact_failed_kernel(const uint8_t* __restrict__ ret, unsigned long long n,
+                                      uint32_t* __restrict__ idx, uint32_t* __restrict__ count,
+                                      uint8_t* __restrict__ fallback)
+{
+    const unsigned long long g = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool bad = g < n && ret[g] != 0;
+    if (g < n && fallback) fallback[g] = bad ? 1 : 0;
+    const unsigned long long m = __ballot(bad);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, 0);
+    if (bad) idx[base + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)g;
+}
+
+}  // namespace
+
+struct wl_decoder {
+    HostCode code;
+    int max_iter = 10;
+    unsigned flags = 0;
+    int frames_per_wg = 0;
+    const wl::DecodeEntry* entry = nullptr;
+    DevBuf<float> offsets;
+    int T = 0;
+    // host-call staging
+    DevBuf<float> d_llr;
+    DevBuf<uint8_t> d_hard, d_ret, d_iters;
+    // fallback scratch
+    DevBuf<uint32_t> d_idx, d_count;
+    hipStream_t stream = nullptr;
+};
+
+struct wl_encoder {
+    HostCode code;
+    wl::EncodeDesc desc{};
+    DevBuf<uint8_t> d_info, d_cw;
+};
+
+extern "C" {
+
+const char* wl_last_error(void) { return g_err.c_str(); }
+
+const char* wl_version(void) { return WL_VERSION_STR; }
+
+int wl_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int wl_code_params(int rate, int z, wl_code_info* out)
+{
+    if (!out) return fail(WL_EINVAL, "null output");
+    HostCode hc;
+    int rc = make_code(rate, z, hc);
+    if (rc) return rc;
+    *out = hc.info;
+    return WL_OK;
+}
+
+int wl_code_table(int rate, int z, int16_t* out, size_t out_len)
+{
+    HostCode hc;
+    int rc = make_code(rate, z, hc);
+    if (rc) return rc;
+    if (!out || out_len < hc.table.size())
+        return fail(WL_EINVAL, "table needs %zu entries", hc.table.size());
+    memcpy(out, hc.table.data(), hc.table.size() * sizeof(int16_t));
+    return WL_OK;
+}
+
+int wl_decoder_create(int rate, int z, int max_iter, unsigned flags, wl_decoder** out)
+{
+    if (!out) return fail(WL_EINVAL, "null output");
+    *out = nullptr;
+    if (max_iter < 0 || max_iter > 255) return fail(WL_EINVAL, "max_iter %d out of range 0..255", max_iter);
+    wl_decoder* d = new (std::nothrow) wl_decoder();
+    if (!d) return fail(WL_ENOMEM, "out of host memory");
+    int rc = make_code(rate, z, d->code);
+    if (rc) {
+        delete d;
+        return rc;
+    }
+    d->max_iter = max_iter;
+    d->flags = flags;
+    d->entry = wl::decode_entry(rate, z);
+    d->frames_per_wg = default_frames_per_wg(z, d->entry->fmax);
+    *out = d;
+    return WL_OK;
+}
+
+int wl_decoder_destroy(wl_decoder* dec)
+{
+    delete dec;
+    return WL_OK;
+}
+
+int wl_decoder_set_frames_per_wg(wl_decoder* dec, int f)
+{
+    if (!dec) return fail(WL_EINVAL, "null decoder");
+    if (f == 0) f = default_frames_per_wg(dec->code.info.z, dec->entry->fmax);
+    if (f < 1 || f > dec->entry->fmax)
+        return fail(WL_EINVAL, "frames_per_wg %d out of range 1..%d", f, dec->entry->fmax);
+    dec->frames_per_wg = f;
+    return WL_OK;
+}
+
+int wl_decoder_get_frames_per_wg(const wl_decoder* dec, int* out)
+{
+    if (!dec || !out) return fail(WL_EINVAL, "null argument");
+    *out = dec->frames_per_wg;
+    return WL_OK;
+}
+
+int wl_decoder_set_offsets(wl_decoder* dec, const float* offsets_host, int T)
+{
+    if (!dec) return fail(WL_EINVAL, "null decoder");
+    if (!offsets_host) {
+        dec->T = 0;
+        return WL_OK;
+    }
+    if (T < 1 || T > 255) return fail(WL_EINVAL, "T %d out of range 1..255", T);
+    const size_t n = (size_t)T * dec->code.info.mb * dec->code.info.c;
+    int rc = dec->offsets.ensure(n);
+    if (rc) return rc;
+    WL_HIP(hipMemcpy(dec->offsets.p, offsets_host, n * sizeof(float), hipMemcpyHostToDevice));
+    dec->T = T;
+    return WL_OK;
+}
+
+static int decode_launch(wl_decoder* dec, bool learned, int max_iter, bool early_stop,
+                         const float* llr, size_t nframes, float* post, uint8_t* hard, uint8_t* ret,
+                         uint8_t* iters, uint16_t* synd, const uint32_t* idx, const uint32_t* count,
+                         hipStream_t s)
+{
+    wl::DecodeArgs a{};
+    a.llr = llr;
+    a.post = post;
+    a.hard = hard;
+    a.ret = ret;
+    a.iters = iters;
+    a.synd = synd;
+    a.offsets = dec->offsets.p;
+    a.frame_idx = idx;
+    a.count_dev = count;
+    a.nframes = nframes;
+    a.frames_per_wg = dec->frames_per_wg;
+    a.max_iter = max_iter;
+    a.early_stop = early_stop ? 1 : 0;
+    a.n_offset_iters = dec->T;
+    hipError_t e = learned ? dec->entry->learned(a, s) : dec->entry->nms(a, s);
+    if (e != hipSuccess) return hip_fail(e, "decoder kernel launch");
+    return WL_OK;
+}
+
+int wl_decode_f32_dev(wl_decoder* dec, const float* llr, size_t nframes, float* post,
+                      uint8_t* hard, uint8_t* ret, uint8_t* iters, uint16_t* synd, void* stream)
+{
+    if (!dec) return fail(WL_EINVAL, "null decoder");
+    if (nframes == 0) return WL_OK;
+    if (!llr) return fail(WL_EINVAL, "null llr");
+    if (nframes > 0xffffffffull) return fail(WL_EINVAL, "nframes %zu too large", nframes);
+    const bool learned = dec->T > 0;
+    return decode_launch(dec, learned, dec->max_iter, (dec->flags & WL_FLAG_EARLY_STOP) != 0, llr,
+                         nframes, post, hard, ret, iters, synd, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nframes,
+                                   int fallback_iters, float* post, uint8_t* hard, uint8_t* ret,
+                                   uint8_t* iters, uint16_t* synd, uint8_t* fallback, void* stream)
+{
+    if (!dec) return fail(WL_EINVAL, "null decoder");
+    if (dec->T <= 0) return fail(WL_EINVAL, "no learned offsets set");
+    if (nframes == 0) return WL_OK;
+    if (!llr || !ret) return fail(WL_EINVAL, "llr and ret are required");
+    if (fallback_iters < 0 || fallback_iters > 255) return fail(WL_EINVAL, "fallback_iters out of range");
+    if (nframes > 0xffffffffull) return fail(WL_EINVAL, "nframes %zu too large", nframes);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = decode_launch(dec, true, dec->max_iter, (dec->flags & WL_FLAG_EARLY_STOP) != 0, llr, nframes,
+                           post, hard, ret, iters, synd, nullptr, nullptr, s);
+    if (rc) return rc;
+    if ((rc = dec->d_idx.ensure(nframes))) return rc;
+    if ((rc = dec->d_count.ensure(1))) return rc;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, dec->d_count.p);
+    const unsigned blocks = (unsigned)((nframes + 255) / 256);
+    hipLaunchKernelGGL(compact_failed_kernel, dim3(blocks), dim3(256), 0, s, ret,
+                       (unsigned long long)nframes, dec->d_idx.p, dec->d_count.p, fallback);
+    WL_HIP(hipGetLastError());
+    return decode_launch(dec, false, fallback_iters, true, llr, nframes, post, hard, ret, iters, synd,
+                         dec->d_idx.p, dec->d_count.p, s);
+}
+
+int wl_decode_f32_host(wl_decoder* dec, float* rx, size_t nframes, int8_t* decoded, uint8_t* ret,
+                       uint8_t* iters)
+{
+    if (!dec) return fail(WL_EINVAL, "null decoder");
+    if (nframes == 0) return WL_OK;
+    if (!rx || !decoded || !ret) return fail(WL_EINVAL, "null argument");
+    const int n = dec->code.info.n;
+    const size_t nb = (size_t)n / 8;
+    int rc;
+    if ((rc = dec->d_llr.ensure(nframes * n)) || (rc = dec->d_hard.ensure(nframes * nb)) ||
+        (rc = dec->d_ret.ensure(nframes)) || (rc = dec->d_iters.ensure(nframes)))
+        return rc;
+    WL_HIP(hipMemcpy(dec->d_llr.p, rx, nframes * n * sizeof(float), hipMemcpyHostToDevice));
+    rc = wl_decode_f32_dev(dec, dec->d_llr.p, nframes, dec->d_llr.p /* in place, as decode() */,
+                           dec->d_hard.p, dec->d_ret.p, dec->d_iters.p, nullptr, nullptr);
+    if (rc) return rc;
+    WL_HIP(hipDeviceSynchronize());
+    std::vector<uint8_t> hb(nframes * nb);
+    WL_HIP(hipMemcpy(rx, dec->d_llr.p, nframes * n * sizeof(float), hipMemcpyDeviceToHost));
+    WL_HIP(hipMemcpy(hb.data(), dec->d_hard.p, hb.size(), hipMemcpyDeviceToHost));
+    WL_HIP(hipMemcpy(ret, dec->d_ret.p, nframes, hipMemcpyDeviceToHost));
+    if (iters) WL_HIP(hipMemcpy(iters, dec->d_iters.p, nframes, hipMemcpyDeviceToHost));
+    for (size_t f = 0; f < nframes; f++)
+        for (int v = 0; v < n; v++) decoded[f * n + v] = (int8_t)((hb[f * nb + v / 8] >> (v % 8)) & 1);
+    return WL_OK;
+}
+
+int wl_decode_i8_host(wl_decoder* dec, const int8_t* rx, size_t nframes, int8_t* decoded, uint8_t* ret,
+                      uint8_t* iters)
+{
+    if (!dec) return fail(WL_EINVAL, "null decoder");
+    if (nframes == 0) return WL_OK;
+    if (!rx) return fail(WL_EINVAL, "null rx");
+    const int n = dec->code.info.n;
+    std::vector<float> soft(nframes * n);
+    for (size_t i = 0; i < soft.size(); i++) soft[i] = (float)(rx[i] * 2.0 - 1.0);  // :115
+    return wl_decode_f32_host(dec, soft.data(), nframes, decoded, ret, iters);
+}
+
+int wl_compute_syndrome_host(const wl_decoder* dec, const int8_t* hd, int exit_early, unsigned* out)
+{
+    if (!dec || !hd || !out) return fail(WL_EINVAL, "null argument");
+    const wl_code_info& ci = dec->code.info;
+    unsigned errors = 0;
+    for (int i = 0; i < ci.m; i++) {
+        int8_t x = 0;
+        for (int k = 0; k < ci.c; k++) {
+            const int16_t v = dec->code.table[(size_t)i * ci.c + k];
+            if (v != -1) x ^= hd[v];
+        }
+        if (x != 0) {
+            if (exit_early) {
+                errors = 1;
+                break;
+            }
+            errors++;
+        }
+    }
+    *out = errors;
+    return WL_OK;
+}
+
+int wl_encoder_create(int rate, int z, wl_encoder** out)
+{
+    if (!out) return fail(WL_EINVAL, "null output");
+    *out = nullptr;
+    wl_encoder* e = new (std::nothrow) wl_encoder();
+    if (!e) return fail(WL_ENOMEM, "out of host memory");
+    int rc = make_code(rate, z, e->code);
+    if (rc) {
+        delete e;
+        return rc;
+    }
+    e->desc = make_encode_desc(e->code);
+    *out = e;
+    return WL_OK;
+}
+
+int wl_encoder_destroy(wl_encoder* enc)
+{
+    delete enc;
+    return WL_OK;
+}
+
+int wl_encode_u8_dev(wl_encoder* enc, const uint8_t* info, size_t nframes, uint8_t* codeword, void* stream)
+{
+    if (!enc) return fail(WL_EINVAL, "null encoder");
+    if (nframes == 0) return WL_OK;
+    if (!info || !codeword) return fail(WL_EINVAL, "null buffer");
+    if (nframes > 0xffffffffull) return fail(WL_EINVAL, "nframes %zu too large", nframes);
+    hipError_t e = wl::launch_encode(enc->desc, info, codeword, nframes, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "encoder kernel launch");
+    return WL_OK;
+}
+
+int wl_encode_u8_host(wl_encoder* enc, const uint8_t* info, size_t nframes, uint8_t* codeword)
+{
+    if (!enc) return fail(WL_EINVAL, "null encoder");
+    if (nframes == 0) return WL_OK;
+    if (!info || !codeword) return fail(WL_EINVAL, "null buffer");
+    const size_t k = enc->code.info.k, n = enc->code.info.n;
+    int rc;
+    if ((rc = enc->d_info.ensure(nframes * k)) || (rc = enc->d_cw.ensure(nframes * n))) return rc;
+    WL_HIP(hipMemcpy(enc->d_info.p, info, nframes * k, hipMemcpyHostToDevice));
+    rc = wl_encode_u8_dev(enc, enc->d_info.p, nframes, enc->d_cw.p, nullptr);
+    if (rc) return rc;
+    WL_HIP(hipDeviceSynchronize());
+    WL_HIP(hipMemcpy(codeword, enc->d_cw.p, nframes * n, hipMemcpyDeviceToHost));
+    return WL_OK;
+}
+
+int wl_channel_dev(wl_encoder* enc, uint64_t seed, uint64_t frame_offset, size_t nframes, float ebn0_lo,
+                   float ebn0_hi, float* llr, uint8_t* bits, float* ebn0, void* stream)
+{
+    if (!enc) return fail(WL_EINVAL, "null encoder");
+    if (nframes == 0) return WL_OK;
+    if (nframes > 0xffffffffull) return fail(WL_EINVAL, "nframes %zu too large", nframes);
+    hipError_t e = wl::launch_channel(enc->desc, seed, frame_offset, nframes, ebn0_lo, ebn0_hi, llr, bits,
+                                      ebn0, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "channel kernel launch");
+    return WL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,60 @@
+// dispatch.hpp -- run-time (rate, z) -> compiled kernel tables.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "qc_code.hpp"
+#include "tdmp.hpp"
+
+namespace wl {
+
+using DecodeLaunch = hipError_t (*)(const DecodeArgs&, hipStream_t);
+
+struct DecodeEntry {
+    DecodeLaunch nms;       // normalised min-sum, the reference's arithmetic
+    DecodeLaunch learned;   // learned per-block offset min-sum (FP32)
+    int fmax;               // largest frames-per-workgroup the kernel accepts
+};
+
+// One table of kNumZ entries per rate, defined in decode_r<rate>.hip.
+const DecodeEntry* decode_table_r0();
+const DecodeEntry* decode_table_r1();
+const DecodeEntry* decode_table_r2();
+const DecodeEntry* decode_table_r3();
+const DecodeEntry* decode_table_r4();
+const DecodeEntry* decode_table_r5();
+
+inline const DecodeEntry* decode_entry(int rate, int z)
+{
+    if (rate < 0 || rate >= kNumRates || !valid_z(z)) return nullptr;
+    const DecodeEntry* t = nullptr;
+    switch (rate) {
+    case 0: t = decode_table_r0(); break;
+    case 1: t = decode_table_r1(); break;
+    case 2: t = decode_table_r2(); break;
+    case 3: t = decode_table_r3(); break;
+    case 4: t = decode_table_r4(); break;
+    default: t = decode_table_r5(); break;
+    }
+    return &t[index_of_z(z)];
+}
+
+template <int RATE, int... ZI>
+constexpr auto make_decode_table(std::integer_sequence<int, ZI...>)
+{
+    struct Arr { DecodeEntry e[sizeof...(ZI)]; };
+    return Arr{{DecodeEntry{&launch_tdmp<RATE, z_of_index(ZI), MODE_NMS>,
+                            &launch_tdmp<RATE, z_of_index(ZI), MODE_LEARNED>,
+                            max_frames_per_wg<RATE, z_of_index(ZI)>()}...}};
+}
+
+}  // namespace wl
+
+#define WL_DEFINE_DECODE_TABLE(RATE)                                                        \
+    namespace wl {                                                                          \
+    const DecodeEntry* decode_table_r##RATE()                                               \
+    {                                                                                       \
+        static const auto tab = make_decode_table<RATE>(std::make_integer_sequence<int, kNumZ>{}); \
+        return tab.e;                                                                       \
+    }                                                                                       \
+    }

@@ -1,0 +1,205 @@
+// encoder.hip -- dual-diagonal QC-LDPC encoder and synthetic channel kernels.
+//
+// encode_kernel restates ldpc_encoder::encode_data (lib/ldpc_encoder.cc:50-105),
+// the Lin-Wei-Ku parity prediction and correction, block-parallel: lane t of
+// a frame owns position t of every z-block, so
+//   V_q[t]  = XOR over info slots (l = q, blk < kb) of u[blk*z + (t + s) mod z]  (:29-48)
+//   P_q     = V_q                       for the last block q = mb-1             (:61-64)
+//   P_q     = P_{q-1} ^ V_{q-1}         for q = 1 .. X  (P_0 = 0)              (:67-70)
+//   P_q     = V_q ^ P_{q+1}             for q = mb-2 down to X+1               (:73-76)
+//   P_0     = V_X ^ P_{X+1} ^ P_X                                              (:79-88)
+//   c[k + q z + t] = P_q[t] ^ P_0[(t + D) mod z] for q >= 1, P_0[t] for q = 0 (:93-103)
+// and the systematic part is the info bytes as given (:90).  All integer XOR,
+// bit-exact with the reference.
+//
+// channel_kernel fuses a seeded source, the encoder and a BPSK/AWGN channel,
+// the GPU counterpart of lib/BER_runner.cc:162-191 (which draws its bits from
+// an LFSR and its noise from std::mt19937; those streams are not reproduced).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "encoder.hpp"
+#include "philox.hpp"
+
+namespace wl {
+
+// Shared encoder body: info bytes (already 0/1) of frame slot fs in LDS at
+// u, P_0 scratch at p0 (z bytes); lane t < z.  Writes the parity bits through
+// put(i, bit) for i in [0, m).
+template <class Put>
+__device__ __forceinline__ void encode_frame(const EncodeDesc& d, const uint8_t* u, uint8_t* p0,
+                                             int t, bool active, Put&& put)
+{
+    const int z = d.z;
+    uint8_t V[kMaxLayers];
+    uint8_t P[kMaxLayers];
+#pragma unroll
+    for (int q = 0; q < kMaxLayers; q++) {
+        uint8_t v = 0;
+        if (q < d.mb && active) {
+            for (int j = 0; j < d.deg[q]; j++) {
+                const int b = d.blk[q][j];
+                if (b >= d.kb) break;            // slots ascend by column: info blocks first
+                int p = t + d.sh[q][j];
+                p = p >= z ? p - z : p;
+                v ^= u[b * z + p];
+            }
+        }
+        V[q] = v;
+        P[q] = 0;
+    }
+    const int mb = d.mb, X = d.x;
+#pragma unroll
+    for (int q = 0; q < kMaxLayers; q++)
+        if (q == mb - 1) P[q] = V[q];
+#pragma unroll
+    for (int q = 1; q < kMaxLayers; q++)
+        if (q <= X) P[q] = P[q - 1] ^ V[q - 1];
+#pragma unroll
+    for (int q = kMaxLayers - 2; q >= 1; q--)
+        if (q <= mb - 2 && q >= X + 1) P[q] = V[q] ^ P[q + 1];
+    uint8_t vx = 0, px1 = 0, px = 0;
+#pragma unroll
+    for (int q = 0; q < kMaxLayers; q++) {
+        if (q == X) { vx = V[q]; px = P[q]; }
+        if (q == X + 1) px1 = P[q];
+    }
+    const uint8_t pz = vx ^ px1 ^ px;
+    if (active) p0[t] = pz;
+    __syncthreads();
+    if (active) {
+        int tr = t + d.d;
+        tr %= z;
+        const uint8_t corr = p0[tr];
+        put(t, pz);
+#pragma unroll
+        for (int q = 1; q < kMaxLayers; q++)
+            if (q < mb) put(q * z + t, (uint8_t)(P[q] ^ corr));
+    }
+}
+
+__global__ __launch_bounds__(256) void encode_kernel(EncodeDesc d, const uint8_t* __restrict__ info,
+                                                     uint8_t* __restrict__ cw,
+                                                     unsigned long long nframes)
+{
+    extern __shared__ uint8_t esm[];
+    const int z = d.z, k = d.k, n = d.n, F = d.frames_per_wg;
+    const unsigned long long frame0 = (unsigned long long)blockIdx.x * F;
+    const unsigned long long left = nframes - frame0;
+    const int Feff = left < (unsigned long long)F ? (int)left : F;
+    const int tid = threadIdx.x;
+    const int fs = tid / z, t = tid - fs * z;
+    const bool active = fs < Feff;
+    uint8_t* u = esm;                         // [F][k]
+    uint8_t* p0 = esm + (size_t)F * k;        // [F][z]
+    const uint8_t* src = info + frame0 * k;
+    uint8_t* dst = cw + frame0 * n;
+    for (int i = tid; i < Feff * k; i += blockDim.x) {
+        const uint8_t b = src[i];
+        u[i] = b != 0;
+        const int f = i / k, j = i - f * k;
+        dst[(size_t)f * n + j] = b;           // systematic part copied as is
+    }
+    __syncthreads();
+    const int fsa = active ? fs : 0;
+    encode_frame(d, u + (size_t)fsa * k, p0 + (size_t)fsa * z, t, active,
+                 [&](int i, uint8_t bit) { dst[(size_t)fs * n + k + i] = bit; });
+}
+
+__global__ __launch_bounds__(256) void channel_kernel(EncodeDesc d, unsigned long long seed,
+                                                      unsigned long long frame_offset,
+                                                      unsigned long long nframes, float eb_lo,
+                                                      float eb_hi, float* __restrict__ llr,
+                                                      uint8_t* __restrict__ bits,
+                                                      float* __restrict__ ebn0_out)
+{
+    extern __shared__ uint8_t esm[];
+    const int z = d.z, k = d.k, n = d.n, F = d.frames_per_wg;
+    const unsigned long long frame0 = (unsigned long long)blockIdx.x * F;
+    const unsigned long long left = nframes - frame0;
+    const int Feff = left < (unsigned long long)F ? (int)left : F;
+    const int tid = threadIdx.x;
+    const int fs = tid / z, t = tid - fs * z;
+    const bool active = fs < Feff;
+    uint8_t* c = esm;                         // [F][n] codeword bits
+    uint8_t* p0 = esm + (size_t)F * n;        // [F][z]
+    const uint32_t key0 = (uint32_t)seed, key1 = (uint32_t)(seed >> 32);
+    // info bits: 128 per Philox block, stream 0
+    for (int i = tid; i < Feff * ((k + 127) / 128); i += blockDim.x) {
+        const int nb = (k + 127) / 128;
+        const int f = i / nb, blk = i - f * nb;
+        const unsigned long long g = frame_offset + frame0 + f;
+        const uint4 r = philox4x32_10(make_uint4((uint32_t)g, (uint32_t)(g >> 32), (uint32_t)blk, 0u),
+                                      key0, key1);
+        const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+        for (int q = 0; q < 128; q++) {
+            const int j = blk * 128 + q;
+            if (j < k) c[(size_t)f * n + j] = (w[q >> 5] >> (q & 31)) & 1u;
+        }
+    }
+    __syncthreads();
+    const int fsa = active ? fs : 0;
+    encode_frame(d, c + (size_t)fsa * n, p0 + (size_t)fsa * z, t, active,
+                 [&](int i, uint8_t bit) { c[(size_t)fs * n + k + i] = bit; });
+    __syncthreads();
+    const float rate = (float)k / (float)n;
+    for (int f = 0; f < Feff; f++) {
+        const unsigned long long g = frame_offset + frame0 + f;
+        float eb = eb_lo;
+        if (eb_hi > eb_lo) {
+            const uint4 r = philox4x32_10(make_uint4((uint32_t)g, (uint32_t)(g >> 32), 0u, 2u), key0, key1);
+            eb = eb_lo + (eb_hi - eb_lo) * (float)u01(r.x);
+        }
+        const double s2 = 1.0 / (2.0 * (double)rate * pow(10.0, (double)eb / 10.0));
+        const double sig = sqrt(s2);
+        if (tid == 0 && ebn0_out) ebn0_out[frame0 + f] = eb;
+        for (int v2 = tid; v2 < n / 2; v2 += blockDim.x) {
+            const uint4 r = philox4x32_10(make_uint4((uint32_t)g, (uint32_t)(g >> 32), (uint32_t)v2, 1u),
+                                          key0, key1);
+            double n0, n1;
+            box_muller(r.x, r.y, n0, n1);
+            const int v = 2 * v2;
+            const uint8_t b0 = c[(size_t)f * n + v], b1 = c[(size_t)f * n + v + 1];
+            const double y0 = (b0 ? -1.0 : 1.0) + sig * n0;
+            const double y1 = (b1 ? -1.0 : 1.0) + sig * n1;
+            if (llr) {
+                // LLR = 2y/sigma^2 for bit 0 -> +1, negated to the reference convention
+                float2 o;
+                o.x = (float)(-2.0 * y0 / s2);
+                o.y = (float)(-2.0 * y1 / s2);
+                reinterpret_cast<float2*>(llr + (frame0 + f) * n)[v2] = o;
+            }
+            if (bits) {
+                bits[(frame0 + f) * n + v] = b0;
+                bits[(frame0 + f) * n + v + 1] = b1;
+            }
+        }
+    }
+}
+
+hipError_t launch_encode(const EncodeDesc& d, const uint8_t* info, uint8_t* cw,
+                         unsigned long long nframes, hipStream_t s)
+{
+    if (nframes == 0) return hipSuccess;
+    const int F = d.frames_per_wg;
+    const unsigned long long grid = (nframes + F - 1) / F;
+    const size_t lds = (size_t)F * (d.k + d.z);
+    hipLaunchKernelGGL(encode_kernel, dim3((unsigned)grid), dim3(((F * d.z + 63) / 64) * 64), lds, s,
+                       d, info, cw, nframes);
+    return hipGetLastError();
+}
+
+hipError_t launch_channel(const EncodeDesc& d, unsigned long long seed, unsigned long long frame_offset,
+                          unsigned long long nframes, float eb_lo, float eb_hi, float* llr,
+                          uint8_t* bits, float* ebn0, hipStream_t s)
+{
+    if (nframes == 0) return hipSuccess;
+    const int F = d.frames_per_wg;
+    const unsigned long long grid = (nframes + F - 1) / F;
+    const size_t lds = (size_t)F * (d.n + d.z);
+    hipLaunchKernelGGL(channel_kernel, dim3((unsigned)grid), dim3(((F * d.z + 63) / 64) * 64), lds, s,
+                       d, seed, frame_offset, nframes, eb_lo, eb_hi, llr, bits, ebn0);
+    return hipGetLastError();
+}
+
+}  // namespace wl

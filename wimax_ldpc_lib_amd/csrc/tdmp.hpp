@@ -1,0 +1,285 @@
+// tdmp.hpp -- layered (TDMP) min-sum QC-LDPC decoder kernel for gfx950.
+//
+// Restates the decoding loop of the reference (lib/ldpc_decoder.cc:122-262)
+// on the GPU, bit-exact:
+//   * one lane owns expanded row r of every layer of one frame; the z rows of a
+//     layer are independent (disjoint variables), so a layer is one parallel
+//     step and layers follow in base-row order, as the row loop at :143 does;
+//   * the frame's posterior LLRs stay in LDS for all iterations; slot j of row
+//     r reads variable blk*z + (r + s) mod z, i.e. rotated LDS indexing with
+//     the shift folded into the instruction offset;
+//   * the check-to-variable messages of a lane's rows live in VGPRs
+//     (sum of layer degrees per lane, 76..88 floats);
+//   * min1 / min2 / sign parity are reduced in registers per lane: min2 via
+//     med3(min1, min2, |L|), and an edge receives min2 iff |L| == min1, which
+//     equals the reference's first-strict-argmin choice (ties give min2 == min1);
+//   * the normalised message step runs in FP64 exactly as the reference's
+//     float*double expression (:220-223): one rounding to float of
+//     s*(min*0.815) for the message and of (double)L + s*(min*0.815) for the
+//     posterior;
+//   * HBM is touched only by the coalesced LLR load, the packed hard-decision
+//     store (ballot of rx >= 0, the VOLK slicer contract at :130,242) and the
+//     optional posterior / per-frame status stores.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qc_code.hpp"
+
+namespace wl {
+
+struct DecodeArgs {
+    const float* llr;          // [nframes][n] input LLRs, reference convention (>= 0 -> bit 1)
+    float* post;               // [nframes][n] posteriors out (nullable)
+    uint8_t* hard;             // [nframes][n/8] hard decisions, bit v%8 of byte v/8 (nullable)
+    uint8_t* ret;              // [nframes] ldpc_decoder::decode return value (nullable)
+    uint8_t* iters;            // [nframes] iterations run (nullable)
+    uint16_t* synd;            // [nframes] failing parity checks of the output (nullable)
+    const float* offsets;      // learned mode: [T][NL][C] per-block offsets
+    const uint32_t* frame_idx; // nullable: slot i decodes frame frame_idx[i] (in and out)
+    const uint32_t* count_dev; // with frame_idx: number of valid slots, read on the device
+    unsigned long long nframes;
+    int frames_per_wg;         // F: frames decoded together by one workgroup
+    int max_iter;
+    int early_stop;            // 1: syndrome check before/after every iteration (as shipped)
+    int n_offset_iters;        // T (learned mode)
+};
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kLdsPad = 512;   // per-frame counters; also keeps wrapped row bases >= 0
+constexpr int kMaxThreads = 512;
+constexpr int kMaxLdsBytes = 65536;
+
+enum { MODE_NMS = 0, MODE_LEARNED = 1 };
+
+template <int RATE, int Z>
+constexpr int max_frames_per_wg()
+{
+    int f = kMaxThreads / Z;
+    while (f > 1 && kLdsPad + f * 24 * Z * 4 > kMaxLdsBytes) f--;
+    return f;
+}
+
+template <int RATE, int Z, int MODE>
+__global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
+{
+    using CD = Code<RATE, Z>;
+    constexpr int NL = CD::NL;
+    constexpr int N = CD::N;
+    constexpr int C = CD::C;
+
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(smem);
+    float* rxall = reinterpret_cast<float*>(smem + kLdsPad);
+
+    const int tid = threadIdx.x;
+    const int F = a.frames_per_wg;
+    const unsigned long long frame0 = (unsigned long long)blockIdx.x * (unsigned long long)F;
+    unsigned long long nfr = a.nframes;
+    if (a.frame_idx) {
+        const unsigned long long c = *a.count_dev;
+        nfr = c < nfr ? c : nfr;
+    }
+    if (frame0 >= nfr) return;   // whole workgroup: no barrier has been reached yet
+    const unsigned long long left = nfr - frame0;
+    const int Feff = left < (unsigned long long)F ? (int)left : F;
+    const int f = tid / Z;
+    const int r = tid - f * Z;
+    const bool active = f < Feff;
+    // global frame of slot s
+    auto gframe = [&](int s) -> unsigned long long {
+        return a.frame_idx ? (unsigned long long)a.frame_idx[frame0 + s] : frame0 + s;
+    };
+
+    if (tid < F) cnt[tid] = 0;
+    if (!a.frame_idx) {
+        const f32x4* src = reinterpret_cast<const f32x4*>(a.llr + frame0 * N);
+        f32x4* dst = reinterpret_cast<f32x4*>(rxall);
+        const int total4 = Feff * (N / 4);
+        for (int i = tid; i < total4; i += blockDim.x) dst[i] = __builtin_nontemporal_load(src + i);
+    } else {
+        for (int s = 0; s < Feff; s++) {
+            const f32x4* src = reinterpret_cast<const f32x4*>(a.llr + gframe(s) * N);
+            f32x4* dst = reinterpret_cast<f32x4*>(rxall + s * N);
+            for (int i = tid; i < N / 4; i += blockDim.x) dst[i] = src[i];
+        }
+    }
+    __syncthreads();
+
+    // Row base pointers: slot (l, j) of this lane's row reads
+    // rx[blk*Z + r + s] or, once r + s wraps, rx[blk*Z + r + s - Z].
+    float* const base0 = rxall + (active ? f : 0) * N + r;
+    float* const baseW = base0 - Z;
+
+    auto at = [&](auto Lc, auto Jc) -> float* {
+        constexpr int L = decltype(Lc)::value;
+        constexpr int J = decltype(Jc)::value;
+        constexpr int b = CD::blk(L, J);
+        constexpr int s = CD::sh(L, J);
+        if constexpr (s == 0) {
+            return base0 + b * Z;
+        } else {
+            return ((r >= Z - s) ? baseW : base0) + (b * Z + s);
+        }
+    };
+
+    float msg[NL][C];
+    static_for<0, NL>([&](auto Lc) {
+        static_for<0, C>([&](auto Jc) { msg[decltype(Lc)::value][decltype(Jc)::value] = 0.0f; });
+    });
+
+    // One layer for this lane's row (lib/ldpc_decoder.cc:143-240).
+    auto layer = [&](auto Lc, int t) {
+        constexpr int L = decltype(Lc)::value;
+        constexpr int D = CD::deg(L);
+        float x[D];
+        float mn1 = 10000.0f, mn2 = 10000.0f;
+        uint32_t sx = 0;
+        static_for<0, D>([&](auto Jc) {
+            constexpr int J = decltype(Jc)::value;
+            const float v = *at(Lc, Jc) - msg[L][J];
+            x[J] = v;
+            const float av = fabsf(v);
+            mn2 = __builtin_amdgcn_fmed3f(mn1, mn2, av);
+            mn1 = fminf(mn1, av);
+            sx ^= __float_as_uint(v);
+        });
+        // message sign bit of slot n = signbit(L_n) ^ parity(signbits) ^ (D & 1)
+        const uint32_t kw = (sx ^ ((uint32_t)(D & 1) << 31)) & 0x80000000u;
+        if constexpr (MODE == MODE_NMS) {
+            const double d1 = __dmul_rn((double)mn1, 0.815);
+            const double d2 = __dmul_rn((double)mn2, 0.815);
+            const float f1 = __double2float_rn(d1);
+            const float f2 = __double2float_rn(d2);
+            static_for<0, D>([&](auto Jc) {
+                constexpr int J = decltype(Jc)::value;
+                const float v = x[J];
+                const bool e = fabsf(v) == mn1;
+                const uint32_t sg = (__float_as_uint(v) ^ kw) & 0x80000000u;
+                const double dd = e ? d2 : d1;
+                const double dsg = __hiloint2double(__double2hiint(dd) | (int)sg, __double2loint(dd));
+                msg[L][J] = __uint_as_float(__float_as_uint(e ? f2 : f1) | sg);
+                *at(Lc, Jc) = __double2float_rn(__dadd_rn((double)v, dsg));
+            });
+        } else {
+            const int tt = t < a.n_offset_iters ? t : a.n_offset_iters - 1;
+            const float* beta = a.offsets + ((size_t)tt * NL + L) * C;
+            static_for<0, D>([&](auto Jc) {
+                constexpr int J = decltype(Jc)::value;
+                const float v = x[J];
+                const bool e = fabsf(v) == mn1;
+                const uint32_t sg = (__float_as_uint(v) ^ kw) & 0x80000000u;
+                float q = (e ? mn2 : mn1) - beta[J];
+                q = q > 0.0f ? q : 0.0f;
+                const float m = __uint_as_float(__float_as_uint(q) | sg);
+                msg[L][J] = m;
+                *at(Lc, Jc) = v + m;
+            });
+        }
+    };
+
+    // Failing parity checks among this lane's rows (lib/ldpc_decoder.cc:67-104
+    // over hd = rx >= 0).
+    auto fail_rows = [&]() -> int {
+        int fails = 0;
+        static_for<0, NL>([&](auto Lc) {
+            constexpr int L = decltype(Lc)::value;
+            uint32_t p = 0;
+            static_for<0, CD::deg(L)>([&](auto Jc) { p ^= (*at(Lc, Jc) >= 0.0f) ? 1u : 0u; });
+            fails += (int)p;
+        });
+        return fails;
+    };
+
+    bool done = !active;
+    int it_done = 0;
+    if (a.early_stop) {
+        // initial hard decision + syndrome (lib/ldpc_decoder.cc:130-139)
+        if (active && fail_rows()) cnt[f] = 1;
+        __syncthreads();
+        if (active && cnt[f] == 0) done = true;
+        __syncthreads();
+        if (tid < F) cnt[tid] = 0;
+    }
+    int any = __syncthreads_or(!done);
+    for (int it = 0; it < a.max_iter && any; it++) {
+        static_for<0, NL>([&](auto Lc) {
+            if (!done) layer(Lc, it);
+            __syncthreads();
+        });
+        if (a.early_stop) {
+            // lib/ldpc_decoder.cc:242-251
+            if (!done && fail_rows()) cnt[f] = 1;
+            __syncthreads();
+            if (!done && cnt[f] == 0) {
+                done = true;
+                it_done = it + 1;
+            }
+            any = __syncthreads_or(!done);
+            if (tid < F) cnt[tid] = 0;
+        }
+    }
+    const bool converged = a.early_stop && done && active;
+    if (!converged) it_done = a.max_iter;
+
+    __syncthreads();
+    if (a.synd || a.ret) {
+        if (active) {
+            const int fr = fail_rows();
+            if (fr) atomicAdd(&cnt[f], (uint32_t)fr);
+        }
+    }
+    __syncthreads();
+    if (active && r == 0) {
+        const unsigned long long g = gframe(f);
+        const uint32_t w = cnt[f];
+        if (a.synd) a.synd[g] = (uint16_t)w;
+        if (a.iters) a.iters[g] = (uint8_t)it_done;
+        if (a.ret) {
+            uint8_t rv;
+            if (a.early_stop) rv = (converged || a.max_iter == 0) ? 0 : 1;  // :124,141,261
+            else rv = w != 0;
+            a.ret[g] = rv;
+        }
+    }
+
+    if (a.post) {
+        for (int s = 0; s < Feff; s++) {
+            f32x4* dst = reinterpret_cast<f32x4*>(a.post + gframe(s) * N);
+            const f32x4* src = reinterpret_cast<const f32x4*>(rxall + s * N);
+            for (int i = tid; i < N / 4; i += blockDim.x) __builtin_nontemporal_store(src[i], dst + i);
+        }
+    }
+    if (a.hard) {
+        // per frame: ballots of 64 consecutive variables, bytes written by lanes 0..7
+        constexpr int kChunks = (N + 63) / 64;
+        const int lane = tid & 63;
+        const int nw = blockDim.x >> 6;
+        for (int q = tid >> 6; q < Feff * kChunks; q += nw) {
+            const int s = q / kChunks, c0 = (q - s * kChunks) * 64;
+            const int v = c0 + lane;
+            const bool bit = v < N && rxall[s * N + v] >= 0.0f;
+            const unsigned long long m = __ballot(bit);
+            const int byte = (c0 >> 3) + lane;
+            if (lane < 8 && byte < N / 8) a.hard[gframe(s) * (N / 8) + byte] = (uint8_t)(m >> (8 * lane));
+        }
+    }
+}
+
+template <int RATE, int Z, int MODE>
+hipError_t launch_tdmp(const DecodeArgs& a, hipStream_t stream)
+{
+    const int F = a.frames_per_wg;
+    if (F < 1 || F > max_frames_per_wg<RATE, Z>()) return hipErrorInvalidValue;
+    if (a.nframes == 0) return hipSuccess;
+    const int threads = ((F * Z + 63) / 64) * 64;
+    const size_t lds = kLdsPad + (size_t)F * Code<RATE, Z>::N * sizeof(float);
+    const unsigned long long grid = (a.nframes + F - 1) / F;
+    if (grid > 0x7fffffffull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((tdmp_kernel<RATE, Z, MODE>), dim3((unsigned)grid), dim3(threads), lds, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace wl
