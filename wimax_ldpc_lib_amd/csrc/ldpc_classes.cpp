@@ -1,0 +1,284 @@
+// ldpc_classes.cpp -- the reference's C++ classes (include/ldpc.h,
+// ldpc_decoder.h, ldpc_encoder.h, BER_runner.h) over the C ABI.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "BER_runner.h"
+#include "ldpc.h"
+#include "ldpc_decoder.h"
+#include "ldpc_encoder.h"
+#include "wimax_ldpc_amd.h"
+
+namespace wimax_ldpc_lib {
+
+namespace {
+void check(int rc, const char* what)
+{
+    if (rc != WL_OK) throw std::runtime_error(std::string(what) + ": " + wl_last_error());
+}
+}  // namespace
+
+// ----------------------------------------------------------------- ldpc
+
+ldpc::ldpc()
+    : m_checknode_array(nullptr), m_D(0), m_N(0), m_M(0), m_X(0), m_z(0), m_col_size(0), m_row_size(0),
+      m_num_threads(0)
+{
+}
+
+// lib/ldpc.cc:11-63
+ldpc::ldpc(coderate rate, unsigned int z_factor, unsigned int num_threads) : ldpc()
+{
+    m_num_threads = num_threads;
+    wl_code_info ci;
+    if (wl_code_params((int)rate, (int)z_factor, &ci) != WL_OK) {
+        printf("[!]ldpc - %s\n", wl_last_error());
+        throw std::exception();
+    }
+    m_z = z_factor;
+    m_N = ci.n;
+    m_M = ci.m;
+    m_D = ci.d;
+    m_X = ci.x;
+    m_row_size = ci.m;
+    m_col_size = ci.c;
+    m_table.resize((size_t)ci.m * ci.c);
+    check(wl_code_table((int)rate, (int)z_factor, m_table.data(), m_table.size()), "wl_code_table");
+    m_checknode_array = m_table.data();
+}
+
+ldpc::~ldpc() {}
+
+uint64_t ldpc::get_nanoseconds()
+{
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::high_resolution_clock::now().time_since_epoch())
+        .count();
+}
+
+// ----------------------------------------------------------------- decoder
+
+ldpc_decoder::ldpc_decoder() : m_max_iter(0), m_rate(HALFRATE), m_handle(nullptr) {}
+
+ldpc_decoder::ldpc_decoder(coderate rate, unsigned int z_factor, unsigned int max_iter,
+                           unsigned int num_threads)
+    : ldpc(rate, z_factor, num_threads), m_max_iter(max_iter), m_rate(rate), m_handle(nullptr)
+{
+    check(wl_decoder_create((int)rate, (int)z_factor, (int)max_iter, WL_FLAG_EARLY_STOP, &m_handle),
+          "wl_decoder_create");
+}
+
+ldpc_decoder::~ldpc_decoder()
+{
+    if (m_handle) wl_decoder_destroy(m_handle);
+}
+
+unsigned int ldpc_decoder::compute_syndrome(int8_t* rx_codeword, bool exit_early)
+{
+    unsigned out = 0;
+    check(wl_compute_syndrome_host(m_handle, rx_codeword, exit_early ? 1 : 0, &out), "compute_syndrome");
+    return out;
+}
+
+unsigned int ldpc_decoder::decode(int8_t* rx_codeword, int8_t* decoded)
+{
+    uint8_t ret = 0;
+    check(wl_decode_i8_host(m_handle, rx_codeword, 1, decoded, &ret, nullptr), "decode");
+    return ret;
+}
+
+unsigned int ldpc_decoder::decode(float* rx_codeword, int8_t* decoded)
+{
+    uint8_t ret = 0;
+    check(wl_decode_f32_host(m_handle, rx_codeword, 1, decoded, &ret, nullptr), "decode");
+    return ret;
+}
+
+void ldpc_decoder::decode_batch(float* rx, size_t nframes, int8_t* decoded, uint8_t* ret, uint8_t* iters)
+{
+    std::vector<uint8_t> tmp;
+    if (!ret) {
+        tmp.resize(nframes);
+        ret = tmp.data();
+    }
+    check(wl_decode_f32_host(m_handle, rx, nframes, decoded, ret, iters), "decode_batch");
+}
+
+void ldpc_decoder::set_early_stop(bool early_stop)
+{
+    // a new handle with the other flag; frames_per_wg carries over
+    int f = 0;
+    check(wl_decoder_get_frames_per_wg(m_handle, &f), "set_early_stop");
+    wl_decoder* h = nullptr;
+    check(wl_decoder_create((int)m_rate, (int)m_z, (int)m_max_iter, early_stop ? WL_FLAG_EARLY_STOP : 0, &h),
+          "set_early_stop");
+    wl_decoder_set_frames_per_wg(h, f);
+    wl_decoder_destroy(m_handle);
+    m_handle = h;
+}
+
+// ----------------------------------------------------------------- encoder
+
+ldpc_encoder::ldpc_encoder(coderate rate, unsigned int z_factor, unsigned int num_threads)
+    : ldpc(rate, z_factor, num_threads), m_handle(nullptr)
+{
+    check(wl_encoder_create((int)rate, (int)z_factor, &m_handle), "wl_encoder_create");
+}
+
+ldpc_encoder::~ldpc_encoder()
+{
+    if (m_handle) wl_encoder_destroy(m_handle);
+}
+
+void ldpc_encoder::encode_data(uint8_t* infoword, uint8_t* codeword)
+{
+    check(wl_encode_u8_host(m_handle, infoword, 1, codeword), "encode_data");
+}
+
+void ldpc_encoder::encode_batch(const uint8_t* infoword, size_t nframes, uint8_t* codeword)
+{
+    check(wl_encode_u8_host(m_handle, infoword, nframes, codeword), "encode_batch");
+}
+
+// ----------------------------------------------------------------- BER_runner
+
+#define BITS_PER_SYM 1   // lib/BER_runner.cc:9
+
+// lib/BER_runner.cc:15-87
+BER_runner::BER_runner(coderate rate, unsigned int z_factor, unsigned int num_threads, unsigned int max_iter)
+    : ldpc(rate, z_factor, num_threads), m_encoder(nullptr), m_decoder(nullptr)
+{
+    m_num_bits = 0;
+    m_num_errors = 0;
+    m_codeword_len = (z_factor / 96.0) * BASE_LDPC_BLOCK_LEN;
+    m_dist_awgn = std::normal_distribution<float>(0.0, 1.0);
+    m_generator_awgn = std::mt19937(0);
+    m_lfsr_start_state = 0xdeadbeef;
+    m_lfsr = m_lfsr_start_state;
+    m_lfsr_reset_len = 1152 * 100;
+    m_lfsr_index = 0;
+    switch (rate) {
+    case HALFRATE:
+        m_dataword_len = m_codeword_len / 2;
+        m_float_rate = 0.5;
+        break;
+    case TWOTHIRDSA:
+    case TWOTHIRDSB:
+        m_dataword_len = (m_codeword_len * 2) / 3;
+        m_float_rate = 2.0 / 3.0;
+        break;
+    case THREEQUARTERSA:
+        m_dataword_len = (m_codeword_len * 3) / 4;
+        m_float_rate = 3.0 / 4.0;
+        break;
+    case THREEQUARTERSB:
+        printf("[!]ber_runner - 3/4B Not Implemented\n");
+        throw std::exception();
+    case FIVESIXTHS:
+        m_dataword_len = (m_codeword_len * 5) / 6;
+        m_float_rate = 5.0 / 6.0;
+        break;
+    default:
+        printf("[!]test_encoder - Invalid Coderate: %d\n", rate);
+        throw std::exception();
+    }
+    m_encoder = new ldpc_encoder(rate, z_factor, num_threads);
+    m_decoder = new ldpc_decoder(rate, z_factor, max_iter, num_threads);
+}
+
+BER_runner::~BER_runner()
+{
+    delete m_encoder;
+    delete m_decoder;
+}
+
+void BER_runner::reset_test()
+{
+    m_num_bits = 0;
+    m_num_errors = 0;
+}
+
+uint64_t BER_runner::get_total_num_bits() { return m_num_bits; }
+uint64_t BER_runner::get_total_num_errors() { return m_num_errors; }
+unsigned int BER_runner::get_codeword_len() { return m_codeword_len; }
+unsigned int BER_runner::get_dataword_len() { return m_dataword_len; }
+
+// lib/BER_runner.cc:124-138: x^31 + x^28 + 1
+uint32_t BER_runner::lfsr_get_num()
+{
+    uint32_t bit = ((m_lfsr >> 0) ^ (m_lfsr >> 3)) & 1;
+    m_lfsr = (m_lfsr >> 1) | (bit << 30);
+    m_lfsr_index++;
+    if (m_lfsr_index == m_lfsr_reset_len) {
+        m_lfsr = m_lfsr_start_state;
+        m_lfsr_index = 0;
+    }
+    return bit;
+}
+
+void BER_runner::fill_with_random(uint8_t* buffer, unsigned int buf_len)
+{
+    for (unsigned int i = 0; i < buf_len; i++) buffer[i] = (uint8_t)lfsr_get_num();
+}
+
+// lib/BER_runner.cc:148-160
+unsigned int BER_runner::compare_data(const uint8_t* original_data, const int8_t* decoded_data)
+{
+    unsigned int num_errors = 0;
+    for (unsigned int j = 0; j < m_codeword_len; j++)
+        if (original_data[j] != decoded_data[j]) num_errors++;
+    return num_errors;
+}
+
+// BPSK and AWGN of lib/BER_runner.cc:172-191 for one encoded frame, with the
+// same operand types (float sample + double sqrt(0.5) * float amplitude *
+// float normal, rounded to float).
+void BER_runner::make_frame(double EbNo_dB, uint8_t* dataword, uint8_t* codeword, float* channel)
+{
+    (void)dataword;
+    double EbNo_linear = pow(10.0, (EbNo_dB / 10.0));
+    float noise_amplitude = (1.0 / std::sqrt(EbNo_linear)) * std::sqrt(1.0 / (m_float_rate * BITS_PER_SYM));
+    for (unsigned int k = 0; k < m_codeword_len; k++) channel[k] = codeword[k] * 2 - 1;
+    for (unsigned int j = 0; j < m_codeword_len; j++)
+        channel[j] = channel[j] + std::sqrt(0.5) * noise_amplitude * m_dist_awgn(m_generator_awgn);
+}
+
+// lib/BER_runner.cc:162-204
+double BER_runner::run_iteration(double EbNo_dB)
+{
+    std::vector<uint8_t> data(m_dataword_len), cw(m_codeword_len);
+    std::vector<float> ch(m_codeword_len);
+    std::vector<int8_t> dec(m_codeword_len);
+    fill_with_random(data.data(), m_dataword_len);
+    m_encoder->encode_data(data.data(), cw.data());
+    make_frame(EbNo_dB, data.data(), cw.data(), ch.data());
+    m_decoder->decode(ch.data(), dec.data());
+    m_num_bits += m_dataword_len;
+    m_num_errors += compare_data(cw.data(), dec.data());
+    return (double)m_num_errors / (double)m_num_bits;
+}
+
+double BER_runner::run_batch(double EbNo_dB, size_t nframes)
+{
+    if (nframes == 0) return m_num_bits ? (double)m_num_errors / (double)m_num_bits : 0.0;
+    const size_t k = m_dataword_len, n = m_codeword_len;
+    std::vector<uint8_t> data(nframes * k), cw(nframes * n);
+    std::vector<float> ch(nframes * n);
+    std::vector<int8_t> dec(nframes * n);
+    for (size_t f = 0; f < nframes; f++) fill_with_random(data.data() + f * k, (unsigned)k);
+    m_encoder->encode_batch(data.data(), nframes, cw.data());
+    for (size_t f = 0; f < nframes; f++) make_frame(EbNo_dB, data.data() + f * k, cw.data() + f * n, ch.data() + f * n);
+    m_decoder->decode_batch(ch.data(), nframes, dec.data(), nullptr, nullptr);
+    for (size_t f = 0; f < nframes; f++) {
+        m_num_bits += m_dataword_len;
+        m_num_errors += compare_data(cw.data() + f * n, dec.data() + f * n);
+    }
+    return (double)m_num_errors / (double)m_num_bits;
+}
+
+}  // namespace wimax_ldpc_lib

@@ -54,6 +54,14 @@ constexpr int kMaxLdsBytes = 65536;
 
 enum { MODE_NMS = 0, MODE_LEARNED = 1 };
 
+// Code-shape variants (bit mask), all bit-identical in results:
+//   VAR_NOHOIST  recompute each slot's wrapped LDS address inside the layer
+//                (2 VALU per slot) instead of keeping one address register per
+//                distinct shift live across the iteration loop -> fewer VGPRs
+//   VAR_REREAD   reload rx and recompute L = rx - msg in the second pass
+//                instead of keeping the row's L values live -> fewer VGPRs
+enum { VAR_NOHOIST = 1, VAR_REREAD = 2 };
+
 template <int RATE, int Z>
 constexpr int max_frames_per_wg()
 {
@@ -62,7 +70,7 @@ constexpr int max_frames_per_wg()
     return f;
 }
 
-template <int RATE, int Z, int MODE>
+template <int RATE, int Z, int MODE, int VAR = 0>
 __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
 {
     using CD = Code<RATE, Z>;
@@ -110,7 +118,8 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
 
     // Row base pointers: slot (l, j) of this lane's row reads
     // rx[blk*Z + r + s] or, once r + s wraps, rx[blk*Z + r + s - Z].
-    float* const base0 = rxall + (active ? f : 0) * N + r;
+    float* const frame_base = rxall + (active ? f : 0) * N;
+    float* const base0 = frame_base + r;
     float* const baseW = base0 - Z;
 
     auto at = [&](auto Lc, auto Jc) -> float* {
@@ -124,6 +133,18 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
             return ((r >= Z - s) ? baseW : base0) + (b * Z + s);
         }
     };
+    auto at_r = [&](auto Lc, auto Jc, int rr) -> float* {
+        constexpr int L = decltype(Lc)::value;
+        constexpr int J = decltype(Jc)::value;
+        constexpr int b = CD::blk(L, J);
+        constexpr int s = CD::sh(L, J);
+        float* const b0 = frame_base + rr;
+        if constexpr (s == 0) {
+            return b0 + b * Z;
+        } else {
+            return ((rr >= Z - s) ? b0 - Z : b0) + (b * Z + s);
+        }
+    };
 
     float msg[NL][C];
     static_for<0, NL>([&](auto Lc) {
@@ -134,13 +155,19 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
     auto layer = [&](auto Lc, int t) {
         constexpr int L = decltype(Lc)::value;
         constexpr int D = CD::deg(L);
+        int rr = r;
+        if constexpr ((VAR & VAR_NOHOIST) != 0) asm volatile("" : "+v"(rr));
+        auto atl = [&](auto Lc2, auto Jc2) -> float* {
+            if constexpr ((VAR & VAR_NOHOIST) != 0) return at_r(Lc2, Jc2, rr);
+            else return at(Lc2, Jc2);
+        };
         float x[D];
         float mn1 = 10000.0f, mn2 = 10000.0f;
         uint32_t sx = 0;
         static_for<0, D>([&](auto Jc) {
             constexpr int J = decltype(Jc)::value;
-            const float v = *at(Lc, Jc) - msg[L][J];
-            x[J] = v;
+            const float v = *atl(Lc, Jc) - msg[L][J];
+            if constexpr ((VAR & VAR_REREAD) == 0) x[J] = v;
             const float av = fabsf(v);
             mn2 = __builtin_amdgcn_fmed3f(mn1, mn2, av);
             mn1 = fminf(mn1, av);
@@ -149,33 +176,42 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
         // message sign bit of slot n = signbit(L_n) ^ parity(signbits) ^ (D & 1)
         const uint32_t kw = (sx ^ ((uint32_t)(D & 1) << 31)) & 0x80000000u;
         if constexpr (MODE == MODE_NMS) {
+            // s*(min*0.815) in double: the per-row magnitudes carry the parity
+            // part of the sign; slot n flips it by signbit(L_n).  The stored
+            // message is that double rounded once to float (:220), the
+            // posterior is (double)L + it rounded once (:222-223).
             const double d1 = __dmul_rn((double)mn1, 0.815);
             const double d2 = __dmul_rn((double)mn2, 0.815);
-            const float f1 = __double2float_rn(d1);
-            const float f2 = __double2float_rn(d2);
+            const uint32_t h1 = (uint32_t)__double2hiint(d1) | kw, h2 = (uint32_t)__double2hiint(d2) | kw;
+            const int l1 = __double2loint(d1), l2 = __double2loint(d2);
+            if constexpr ((VAR & VAR_REREAD) != 0) asm volatile("" ::: "memory");
             static_for<0, D>([&](auto Jc) {
                 constexpr int J = decltype(Jc)::value;
-                const float v = x[J];
+                float v;
+                if constexpr ((VAR & VAR_REREAD) != 0) v = *atl(Lc, Jc) - msg[L][J];
+                else v = x[J];
                 const bool e = fabsf(v) == mn1;
-                const uint32_t sg = (__float_as_uint(v) ^ kw) & 0x80000000u;
-                const double dd = e ? d2 : d1;
-                const double dsg = __hiloint2double(__double2hiint(dd) | (int)sg, __double2loint(dd));
-                msg[L][J] = __uint_as_float(__float_as_uint(e ? f2 : f1) | sg);
-                *at(Lc, Jc) = __double2float_rn(__dadd_rn((double)v, dsg));
+                const uint32_t hi = (e ? h2 : h1) ^ (__float_as_uint(v) & 0x80000000u);
+                const double dsg = __hiloint2double((int)hi, e ? l2 : l1);
+                msg[L][J] = __double2float_rn(dsg);
+                *atl(Lc, Jc) = __double2float_rn(__dadd_rn((double)v, dsg));
             });
         } else {
             const int tt = t < a.n_offset_iters ? t : a.n_offset_iters - 1;
             const float* beta = a.offsets + ((size_t)tt * NL + L) * C;
+            if constexpr ((VAR & VAR_REREAD) != 0) asm volatile("" ::: "memory");
             static_for<0, D>([&](auto Jc) {
                 constexpr int J = decltype(Jc)::value;
-                const float v = x[J];
+                float v;
+                if constexpr ((VAR & VAR_REREAD) != 0) v = *atl(Lc, Jc) - msg[L][J];
+                else v = x[J];
                 const bool e = fabsf(v) == mn1;
                 const uint32_t sg = (__float_as_uint(v) ^ kw) & 0x80000000u;
                 float q = (e ? mn2 : mn1) - beta[J];
                 q = q > 0.0f ? q : 0.0f;
                 const float m = __uint_as_float(__float_as_uint(q) | sg);
                 msg[L][J] = m;
-                *at(Lc, Jc) = v + m;
+                *atl(Lc, Jc) = v + m;
             });
         }
     };
@@ -186,8 +222,15 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
         int fails = 0;
         static_for<0, NL>([&](auto Lc) {
             constexpr int L = decltype(Lc)::value;
+            int rr = r;
+            if constexpr ((VAR & VAR_NOHOIST) != 0) asm volatile("" : "+v"(rr));
             uint32_t p = 0;
-            static_for<0, CD::deg(L)>([&](auto Jc) { p ^= (*at(Lc, Jc) >= 0.0f) ? 1u : 0u; });
+            static_for<0, CD::deg(L)>([&](auto Jc) {
+                float* ad;
+                if constexpr ((VAR & VAR_NOHOIST) != 0) ad = at_r(Lc, Jc, rr);
+                else ad = at(Lc, Jc);
+                p ^= (*ad >= 0.0f) ? 1u : 0u;
+            });
             fails += (int)p;
         });
         return fails;
@@ -268,7 +311,7 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
     }
 }
 
-template <int RATE, int Z, int MODE>
+template <int RATE, int Z, int MODE, int VAR = 0>
 hipError_t launch_tdmp(const DecodeArgs& a, hipStream_t stream)
 {
     const int F = a.frames_per_wg;
@@ -278,7 +321,7 @@ hipError_t launch_tdmp(const DecodeArgs& a, hipStream_t stream)
     const size_t lds = kLdsPad + (size_t)F * Code<RATE, Z>::N * sizeof(float);
     const unsigned long long grid = (a.nframes + F - 1) / F;
     if (grid > 0x7fffffffull) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((tdmp_kernel<RATE, Z, MODE>), dim3((unsigned)grid), dim3(threads), lds, stream, a);
+    hipLaunchKernelGGL((tdmp_kernel<RATE, Z, MODE, VAR>), dim3((unsigned)grid), dim3(threads), lds, stream, a);
     return hipGetLastError();
 }
 
