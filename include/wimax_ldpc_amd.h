@@ -87,9 +87,20 @@ int wl_code_table(int rate, int z, int16_t* out, size_t out_len);
 int wl_decoder_create(int rate, int z, int max_iter, unsigned flags, wl_decoder** out);
 int wl_decoder_destroy(wl_decoder* dec);
 
-/* Frames decoded together by one workgroup (0 = autotuned default). */
+/* Launch shape of a decoder (no counterpart in the reference, whose decoder
+ * is one frame per CPU thread).  frames_per_wg: frames decoded together by
+ * one workgroup (0 = the autotuned default); variant: code shape of the
+ * normalised min-sum kernel (0: slot addresses held in registers, 1: slot
+ * addresses recomputed per layer, fewer registers).  Results are identical
+ * for every shape; create() starts from the autotuned table. */
 int wl_decoder_set_frames_per_wg(wl_decoder* dec, int frames_per_wg);
 int wl_decoder_get_frames_per_wg(const wl_decoder* dec, int* out);
+int wl_decoder_set_variant(wl_decoder* dec, int variant);
+int wl_decoder_get_variant(const wl_decoder* dec, int* out);
+
+/* Autotuned launch shape compiled into this build for (rate, z): returns 0,
+ * or 1 when the table has no entry and the heuristic default is reported. */
+int wl_tuned_launch(int rate, int z, int* frames_per_wg, int* variant);
 
 /* Batch decode of device-resident frames.  Replaces the OpenMP frame loop of
  * lib/decode_example.cc:128-135 over ldpc_decoder::decode(float* rx,

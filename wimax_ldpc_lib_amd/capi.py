@@ -26,6 +26,9 @@ SIGNATURES = {
     "wl_decoder_destroy": (C.c_int, [_P]),
     "wl_decoder_set_frames_per_wg": (C.c_int, [_P, C.c_int]),
     "wl_decoder_get_frames_per_wg": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "wl_decoder_set_variant": (C.c_int, [_P, C.c_int]),
+    "wl_decoder_get_variant": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "wl_tuned_launch": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "wl_decode_f32_dev": (C.c_int, [_P, _P, _SZ, _P, _P, _P, _P, _P, _P]),
     "wl_decode_f32_host": (C.c_int, [_P, _P, _SZ, _P, _P, _P]),
     "wl_decode_i8_host": (C.c_int, [_P, _P, _SZ, _P, _P, _P]),

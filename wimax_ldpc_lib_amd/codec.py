@@ -84,6 +84,16 @@ class ldpc_decoder(ldpc):
     def frames_per_wg(self, f):
         check(lib().wl_decoder_set_frames_per_wg(self._h, int(f)))
 
+    @property
+    def variant(self):
+        v = C.c_int()
+        check(lib().wl_decoder_get_variant(self._h, C.byref(v)))
+        return v.value
+
+    @variant.setter
+    def variant(self, v):
+        check(lib().wl_decoder_set_variant(self._h, int(v)))
+
     def set_offsets(self, offsets):
         """Learned offsets [T][mb][c] (numpy float32), or None for plain NMS."""
         if offsets is None:

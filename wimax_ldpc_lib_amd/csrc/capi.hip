@@ -116,6 +116,26 @@ int default_frames_per_wg(int z, int fmax)
     return best;
 }
 
+// Autotuned launch shapes, written by wimax_ldpc_lib_amd/autotune.py from
+// measurements on MI355X: {rate, z, frames_per_wg, variant}.
+struct Tuned {
+    int rate, z, f, variant;
+};
+const Tuned kTuned[] = {
+#include "tuned_table.inc"
+    {-1, 0, 0, 0}};
+
+bool tuned_lookup(int rate, int z, int* f, int* variant)
+{
+    for (const Tuned& t : kTuned)
+        if (t.rate == rate && t.z == z) {
+            *f = t.f;
+            *variant = t.variant;
+            return true;
+        }
+    return false;
+}
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
@@ -163,6 +183,7 @@ struct wl_decoder {
     int max_iter = 10;
     unsigned flags = 0;
     int frames_per_wg = 0;
+    int variant = 0;
     const wl::DecodeEntry* entry = nullptr;
     DevBuf<float> offsets;
     int T = 0;
@@ -232,7 +253,14 @@ int wl_decoder_create(int rate, int z, int max_iter, unsigned flags, wl_decoder*
     d->max_iter = max_iter;
     d->flags = flags;
     d->entry = wl::decode_entry(rate, z);
-    d->frames_per_wg = default_frames_per_wg(z, d->entry->fmax);
+    int tf = 0, tv = 0;
+    if (tuned_lookup(rate, z, &tf, &tv) && tf >= 1 && tf <= d->entry->fmax && tv >= 0 && tv < wl::kNumVariants) {
+        d->frames_per_wg = tf;
+        d->variant = tv;
+    } else {
+        d->frames_per_wg = default_frames_per_wg(z, d->entry->fmax);
+        d->variant = 0;
+    }
     *out = d;
     return WL_OK;
 }
@@ -243,10 +271,44 @@ int wl_decoder_destroy(wl_decoder* dec)
     return WL_OK;
 }
 
+int wl_tuned_launch(int rate, int z, int* frames_per_wg, int* variant)
+{
+    if (!frames_per_wg || !variant) return fail(WL_EINVAL, "null output");
+    HostCode hc;
+    int rc = make_code(rate, z, hc);
+    if (rc) return rc;
+    if (!tuned_lookup(rate, z, frames_per_wg, variant)) {
+        *frames_per_wg = default_frames_per_wg(z, wl::decode_entry(rate, z)->fmax);
+        *variant = 0;
+        return 1;
+    }
+    return WL_OK;
+}
+
+int wl_decoder_set_variant(wl_decoder* dec, int variant)
+{
+    if (!dec) return fail(WL_EINVAL, "null decoder");
+    if (variant < 0 || variant >= wl::kNumVariants)
+        return fail(WL_EINVAL, "variant %d out of range 0..%d", variant, wl::kNumVariants - 1);
+    dec->variant = variant;
+    return WL_OK;
+}
+
+int wl_decoder_get_variant(const wl_decoder* dec, int* out)
+{
+    if (!dec || !out) return fail(WL_EINVAL, "null argument");
+    *out = dec->variant;
+    return WL_OK;
+}
+
 int wl_decoder_set_frames_per_wg(wl_decoder* dec, int f)
 {
     if (!dec) return fail(WL_EINVAL, "null decoder");
-    if (f == 0) f = default_frames_per_wg(dec->code.info.z, dec->entry->fmax);
+    if (f == 0) {
+        int tv = 0;
+        if (!tuned_lookup(dec->code.info.rate, dec->code.info.z, &f, &tv))
+            f = default_frames_per_wg(dec->code.info.z, dec->entry->fmax);
+    }
     if (f < 1 || f > dec->entry->fmax)
         return fail(WL_EINVAL, "frames_per_wg %d out of range 1..%d", f, dec->entry->fmax);
     dec->frames_per_wg = f;
@@ -296,7 +358,7 @@ static int decode_launch(wl_decoder* dec, bool learned, int max_iter, bool early
     a.max_iter = max_iter;
     a.early_stop = early_stop ? 1 : 0;
     a.n_offset_iters = dec->T;
-    hipError_t e = learned ? dec->entry->learned(a, s) : dec->entry->nms(a, s);
+    hipError_t e = learned ? dec->entry->learned(a, s) : dec->entry->nms[dec->variant](a, s);
     if (e != hipSuccess) return hip_fail(e, "decoder kernel launch");
     return WL_OK;
 }

@@ -10,10 +10,12 @@ namespace wl {
 
 using DecodeLaunch = hipError_t (*)(const DecodeArgs&, hipStream_t);
 
+constexpr int kNumVariants = 2;   // code-shape variants of the NMS kernel (tdmp.hpp VAR_*)
+
 struct DecodeEntry {
-    DecodeLaunch nms;       // normalised min-sum, the reference's arithmetic
-    DecodeLaunch learned;   // learned per-block offset min-sum (FP32)
-    int fmax;               // largest frames-per-workgroup the kernel accepts
+    DecodeLaunch nms[kNumVariants];  // normalised min-sum, the reference's arithmetic
+    DecodeLaunch learned;            // learned per-block offset min-sum (FP32)
+    int fmax;                        // largest frames-per-workgroup the kernel accepts
 };
 
 // One table of kNumZ entries per rate, defined in decode_r<rate>.hip.
@@ -43,7 +45,8 @@ template <int RATE, int... ZI>
 constexpr auto make_decode_table(std::integer_sequence<int, ZI...>)
 {
     struct Arr { DecodeEntry e[sizeof...(ZI)]; };
-    return Arr{{DecodeEntry{&launch_tdmp<RATE, z_of_index(ZI), MODE_NMS>,
+    return Arr{{DecodeEntry{{&launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0>,
+                             &launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_NOHOIST>},
                             &launch_tdmp<RATE, z_of_index(ZI), MODE_LEARNED>,
                             max_frames_per_wg<RATE, z_of_index(ZI)>()}...}};
 }
