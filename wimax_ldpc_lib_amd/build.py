@@ -37,7 +37,7 @@ CXX_SOURCES = ["ldpc_classes.cpp"]
 
 
 def _headers():
-    hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h", ".inc"))]
     inc = os.path.join(ROOT, "include")
     hs += [os.path.join(inc, f) for f in os.listdir(inc) if f.endswith(".h")]
     return hs
