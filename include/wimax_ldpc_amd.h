@@ -146,6 +146,41 @@ int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nfr
                                    uint8_t* ret, uint8_t* iters, uint16_t* synd,
                                    uint8_t* fallback, void* stream);
 
+/* ---------------------------------------------------------------- trainer */
+
+/* Learned offset min-sum trainer.  No counterpart in the reference, whose
+ * normalisation / offset constants are fixed by hand (lib/ldpc_decoder.cc:17-21).
+ * Model: T unrolled layered iterations with messages s * max(mag - beta, 0),
+ * one offset per nonzero base-matrix block per iteration (offsets [T][mb][c],
+ * the layout of wl_decoder_set_offsets); loss: mean binary cross-entropy of
+ * the final posteriors over the n coded bits.  Each step draws `batch` fresh
+ * frames on the GPU (wl_channel_dev's generator, seed `seed`, Eb/N0 uniform in
+ * [ebn0_lo, ebn0_hi) per frame), runs the forward and backward kernels and an
+ * Adam step (lr, beta1 0.9, beta2 0.999, eps 1e-8).  Kernels exist for z = 24
+ * and z = 96 (WL_ENOTSUP otherwise) and for the five rates with a valid
+ * encoder (3/4B: WL_ENOTSUP).  All offsets start at init_offset. */
+typedef struct wl_trainer wl_trainer;
+int wl_trainer_create(int rate, int z, int T, int batch, uint64_t seed, float lr, float init_offset,
+                      wl_trainer** out);
+int wl_trainer_destroy(wl_trainer* tr);
+int wl_trainer_set_hparams(wl_trainer* tr, float lr, float beta1, float beta2, float eps);
+/* nsteps training steps (one captured hipGraph launch each); last_loss
+ * (nullable) receives the mean loss of the last step's batch. */
+int wl_trainer_step(wl_trainer* tr, int nsteps, float ebn0_lo, float ebn0_hi, float* last_loss);
+int wl_trainer_num_params(const wl_trainer* tr, int* T, int* mb, int* c);
+int wl_trainer_get_offsets(wl_trainer* tr, float* offsets_host);
+int wl_trainer_set_offsets(wl_trainer* tr, const float* offsets_host);
+int wl_trainer_get_step(wl_trainer* tr, uint64_t* step);
+/* Checkpoint / resume: offsets, Adam moments, step counter and seed. */
+int wl_trainer_save(wl_trainer* tr, const char* path);
+int wl_trainer_load(wl_trainer* tr, const char* path);
+/* Loss (summed over frames and bits) and its gradient with respect to the
+ * current offsets on caller frames: llr [nframes][n] and bits [nframes][n]
+ * on the device; post (nullable, device) gets the final posteriors;
+ * grad_host [T][mb][c] is the gradient of the MEAN loss. */
+int wl_trainer_grad_dev(wl_trainer* tr, const float* llr, const uint8_t* bits, size_t nframes, float* post,
+                        float* grad_host, double* loss_host);
+
 /* ---------------------------------------------------------------- encoder */
 
 /* Replaces ldpc_encoder::ldpc_encoder(rate, z, num_threads)

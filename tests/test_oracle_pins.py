@@ -320,3 +320,59 @@ def test_ber_agrees_with_published_curves(oracle_mod):
         o = code.decode(llr, 10, True)
         ber = (o["hd"] != cw).sum() / (frames * code.k)
         assert published / 6 < ber < published * 2, (rate, eb, ber)
+
+
+# ------------------------------------------------------------------ learned offset min-sum
+
+def test_learned_row_matches_brute_force(oracle_mod):
+    """Learned row: msg = sign(ext) * max(|ext| - beta_j, 0) in FP32."""
+    rng = np.random.default_rng(13)
+    for d in (7, 10, 15):
+        for trial in range(10):
+            L = (rng.uniform(0.05, 6.0, d) * rng.choice([-1.0, 1.0], d)).astype(np.float32)
+            beta = rng.uniform(0.0, 1.0, d).astype(np.float32)
+            ext = _brute_extrinsic(L).astype(np.float32)
+            post, msg = oracle_mod.row_update(oracle_mod.LEARNED, L, beta=beta)
+            mag = np.maximum(np.abs(ext) - beta, np.float32(0.0)).astype(np.float32)
+            want = np.where(ext < 0, -mag, mag).astype(np.float32)
+            assert np.array_equal(np.abs(msg), np.abs(want)) and np.array_equal(np.sign(msg[mag > 0]),
+                                                                                 np.sign(want[mag > 0]))
+            assert np.array_equal(post, (L + msg).astype(np.float32))
+
+
+def test_learned_backward_matches_finite_differences(oracle_mod):
+    """The oracle's exact backward against central differences of its own
+    forward loss (double accumulation) on a small code."""
+    code = oracle_mod.Code(1, 24)
+    llr, cw = noisy_llr(code, 24, 2.0, seed=21)
+    T = 3
+    rng = np.random.default_rng(22)
+    off = rng.uniform(0.05, 0.4, (T, code.mb, code.c)).astype(np.float32)
+    loss, _, g = code.learned_fwd_bwd(off, llr, cw)
+    N = llr.shape[0] * code.n
+    checked = 0
+    for idx in np.argsort(-np.abs(g).ravel())[:12]:
+        t, l, j = np.unravel_index(idx, g.shape)
+        if j >= code.deg[l]:
+            continue
+        h = 5e-4   # float forward: smaller steps hit rounding, larger ones min/clip kinks
+        op, om = off.copy(), off.copy()
+        op[t, l, j] += h
+        om[t, l, j] -= h
+        lp, _, _ = code.learned_fwd_bwd(op, llr, cw)
+        lm, _, _ = code.learned_fwd_bwd(om, llr, cw)
+        fd = (lp - lm) / (2 * h) / N
+        assert abs(fd - g[t, l, j]) <= 0.05 * abs(g[t, l, j]) + 1e-7, (t, l, j, fd, g[t, l, j])
+        checked += 1
+    assert checked >= 8
+
+
+def test_learned_forward_equals_decoder_variant(oracle_mod):
+    """The training forward and the learned decoding variant are the same map:
+    equal final posteriors after T fixed iterations."""
+    code = oracle_mod.Code(3, 24)
+    llr, cw = noisy_llr(code, 8, 2.5, seed=23)
+    off = np.random.default_rng(24).uniform(0.0, 0.5, (4, code.mb, code.c)).astype(np.float32)
+    _, post, _ = code.learned_fwd_bwd(off, llr, cw)
+    o = code.decode(llr, 4, False, variant=oracle_mod.LEARNED, offsets=off)
+    assert np.array_equal(post.view(np.uint32), o["post"].view(np.uint32))

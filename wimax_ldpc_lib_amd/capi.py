@@ -35,6 +35,18 @@ SIGNATURES = {
     "wl_compute_syndrome_host": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_uint)]),
     "wl_decoder_set_offsets": (C.c_int, [_P, _P, C.c_int]),
     "wl_decode_learned_fallback_dev": (C.c_int, [_P, _P, _SZ, C.c_int, _P, _P, _P, _P, _P, _P, _P]),
+    "wl_trainer_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_float, C.c_float,
+                                     C.POINTER(_P)]),
+    "wl_trainer_destroy": (C.c_int, [_P]),
+    "wl_trainer_set_hparams": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "wl_trainer_step": (C.c_int, [_P, C.c_int, C.c_float, C.c_float, C.POINTER(C.c_float)]),
+    "wl_trainer_num_params": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "wl_trainer_get_offsets": (C.c_int, [_P, _P]),
+    "wl_trainer_set_offsets": (C.c_int, [_P, _P]),
+    "wl_trainer_get_step": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "wl_trainer_save": (C.c_int, [_P, C.c_char_p]),
+    "wl_trainer_load": (C.c_int, [_P, C.c_char_p]),
+    "wl_trainer_grad_dev": (C.c_int, [_P, _P, _P, _SZ, _P, _P, C.POINTER(C.c_double)]),
     "wl_encoder_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_P)]),
     "wl_encoder_destroy": (C.c_int, [_P]),
     "wl_encode_u8_dev": (C.c_int, [_P, _P, _SZ, _P, _P]),

@@ -13,6 +13,7 @@
 #include "../../include/wimax_ldpc_amd.h"
 #include "dispatch.hpp"
 #include "encoder.hpp"
+#include "errors.hpp"
 
 #define WL_VERSION_STR "wimax_ldpc_amd 0.1 (gfx950)"
 
@@ -177,6 +178,8 @@ __global__ void compact_failed_kernel(const uint8_t* __restrict__ ret, unsigned 
 }
 
 }  // namespace
+
+void wl::set_error_text(const char* text) { g_err = text; }
 
 struct wl_decoder {
     HostCode code;

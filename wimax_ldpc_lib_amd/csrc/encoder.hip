@@ -111,10 +111,13 @@ __global__ __launch_bounds__(256) void channel_kernel(EncodeDesc d, unsigned lon
                                                       unsigned long long nframes, float eb_lo,
                                                       float eb_hi, float* __restrict__ llr,
                                                       uint8_t* __restrict__ bits,
-                                                      float* __restrict__ ebn0_out)
+                                                      float* __restrict__ ebn0_out,
+                                                      const unsigned long long* __restrict__ step_dev)
 {
     extern __shared__ uint8_t esm[];
     const int z = d.z, k = d.k, n = d.n, F = d.frames_per_wg;
+    // graph replays advance through fresh frames: absolute frame = offset + step * nframes
+    if (step_dev) frame_offset += *step_dev * nframes;
     const unsigned long long frame0 = (unsigned long long)blockIdx.x * F;
     const unsigned long long left = nframes - frame0;
     const int Feff = left < (unsigned long long)F ? (int)left : F;
@@ -191,14 +194,14 @@ hipError_t launch_encode(const EncodeDesc& d, const uint8_t* info, uint8_t* cw,
 
 hipError_t launch_channel(const EncodeDesc& d, unsigned long long seed, unsigned long long frame_offset,
                           unsigned long long nframes, float eb_lo, float eb_hi, float* llr,
-                          uint8_t* bits, float* ebn0, hipStream_t s)
+                          uint8_t* bits, float* ebn0, hipStream_t s, const unsigned long long* step_dev)
 {
     if (nframes == 0) return hipSuccess;
     const int F = d.frames_per_wg;
     const unsigned long long grid = (nframes + F - 1) / F;
     const size_t lds = (size_t)F * (d.n + d.z);
     hipLaunchKernelGGL(channel_kernel, dim3((unsigned)grid), dim3(((F * d.z + 63) / 64) * 64), lds, s,
-                       d, seed, frame_offset, nframes, eb_lo, eb_hi, llr, bits, ebn0);
+                       d, seed, frame_offset, nframes, eb_lo, eb_hi, llr, bits, ebn0, step_dev);
     return hipGetLastError();
 }
 

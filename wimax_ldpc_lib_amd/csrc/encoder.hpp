@@ -21,6 +21,7 @@ hipError_t launch_encode(const EncodeDesc& d, const uint8_t* info, uint8_t* cw,
                          unsigned long long nframes, hipStream_t s);
 hipError_t launch_channel(const EncodeDesc& d, unsigned long long seed, unsigned long long frame_offset,
                           unsigned long long nframes, float eb_lo, float eb_hi, float* llr,
-                          uint8_t* bits, float* ebn0, hipStream_t s);
+                          uint8_t* bits, float* ebn0, hipStream_t s,
+                          const unsigned long long* step_dev = nullptr);
 
 }  // namespace wl
