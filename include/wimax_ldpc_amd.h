@@ -146,6 +146,39 @@ int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nfr
                                    uint8_t* ret, uint8_t* iters, uint16_t* synd,
                                    uint8_t* fallback, void* stream);
 
+/* ---------------------------------------------------------------- mixed batches */
+
+/* Frames of different (rate, z) decoded together (no counterpart in the
+ * reference, where one decoder object serves one code).  A frame's code id is
+ * rate * 19 + (z - 24) / 4 (0..113).  Frame g holds n_g = 24 z_g LLRs starting
+ * at element frame_off[g] of a ragged buffer; its posteriors go to the same
+ * offsets of post, its packed hard decisions to byte frame_off[g] / 8 of hard,
+ * and ret / iters / synd to index g.  A decode is one hipGraph launch: frames
+ * are bucketed by code on the device and each code's bucket is decoded by a
+ * persistent kernel with that code's autotuned launch shape; results equal a
+ * per-code wl_decode_f32_dev bit for bit.  The graph is captured at the first
+ * call with a given set of buffers and replayed afterwards.  Frames whose code
+ * id is invalid (or not enabled, see wl_mixed_set_codes) are skipped. */
+typedef struct wl_mixed wl_mixed;
+int wl_mixed_create(int max_iter, unsigned flags, wl_mixed** out);
+int wl_mixed_destroy(wl_mixed* m);
+/* Restrict the graph to the listed code ids (ncodes = 0: all 114). */
+int wl_mixed_set_codes(wl_mixed* m, const int* codes, int ncodes);
+int wl_mixed_get_launch(const wl_mixed* m, int rate, int z, int* frames_per_wg, int* variant, int* grid);
+/* frame_off [nframes] = exclusive prefix sum of n over cfg [nframes] (device). */
+int wl_mixed_offsets_dev(wl_mixed* m, const uint8_t* cfg, size_t nframes, uint64_t* frame_off, void* stream);
+int wl_mixed_decode_dev(wl_mixed* m, const uint8_t* cfg, const uint64_t* frame_off, const float* llr,
+                        size_t nframes, float* post, uint8_t* hard, uint8_t* ret, uint8_t* iters,
+                        uint16_t* synd, void* stream);
+/* Frames skipped by the last decode for an invalid or disabled code id
+ * (synchronizes the device). */
+int wl_mixed_invalid_count(wl_mixed* m, uint64_t* out);
+/* Seeded synthetic mixed frames: frame g of code cfg[g] is exactly frame g of
+ * wl_channel_dev on an encoder of that code (same seed). */
+int wl_mixed_channel_dev(wl_mixed* m, uint64_t seed, const uint8_t* cfg, const uint64_t* frame_off,
+                         size_t nframes, float ebn0_lo, float ebn0_hi, float* llr, uint8_t* bits, float* ebn0,
+                         void* stream);
+
 /* ---------------------------------------------------------------- trainer */
 
 /* Learned offset min-sum trainer.  No counterpart in the reference, whose

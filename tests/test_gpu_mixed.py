@@ -1,0 +1,135 @@
+"""Mixed-(rate, z) batches: on-device bucketing + one captured graph of
+per-code persistent kernels, checked bit-exact against per-code decoding."""
+import numpy as np
+import pytest
+
+from tests.helpers import bits_equal_f32
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(torch, F, seed, codes, lo=1.0, hi=4.5):
+    from wimax_ldpc_lib_amd.mixed import MixedDecoder
+    rng = np.random.default_rng(seed)
+    cfg_np = rng.choice(np.array(codes, dtype=np.uint8), F)
+    cfg = torch.from_numpy(cfg_np).cuda()
+    md = MixedDecoder(max_iter=10, early_stop=True)
+    off = torch.empty(F, dtype=torch.int64, device="cuda")
+    md.offsets_dev(cfg, off)
+    n_of = np.array([24 * (24 + 4 * (int(c) % 19)) for c in cfg_np], dtype=np.int64)
+    want = np.concatenate([[0], np.cumsum(n_of)[:-1]])
+    assert np.array_equal(off.cpu().numpy(), want)
+    total = int(n_of.sum())
+    llr = torch.empty(total, dtype=torch.float32, device="cuda")
+    bits = torch.empty(total, dtype=torch.uint8, device="cuda")
+    md.channel_dev(seed, cfg, off, lo, hi, llr, bits)
+    torch.cuda.synchronize()
+    return md, cfg_np, cfg, off, want, n_of, llr, bits
+
+
+def _mixed_decode(torch, md, cfg, off, llr):
+    F = cfg.numel()
+    total = llr.numel()
+    post = torch.empty(total, dtype=torch.float32, device="cuda")
+    hard = torch.empty(total // 8, dtype=torch.uint8, device="cuda")
+    ret = torch.empty(F, dtype=torch.uint8, device="cuda")
+    iters = torch.empty(F, dtype=torch.uint8, device="cuda")
+    synd = torch.empty(F, dtype=torch.int16, device="cuda")
+    md.decode_dev(cfg, off, llr, post, hard, ret, iters, synd)
+    torch.cuda.synchronize()
+    return post, hard, ret, iters, synd
+
+
+def test_mixed_channel_matches_per_code_channel(gpu):
+    import wimax_ldpc_lib_amd as W
+    torch = gpu
+    codes = [0 * 19 + 0, 1 * 19 + 9, 5 * 19 + 18, 3 * 19 + 7]
+    md, cfg_np, cfg, off, offs, n_of, llr, bits = _setup(torch, 64, 5, codes, 1.0, 4.5)
+    llr_np, bits_np = llr.cpu().numpy(), bits.cpu().numpy()
+    for c in codes:
+        rate, z = int(c) // 19, 24 + 4 * (int(c) % 19)
+        enc = W.ldpc_encoder(rate, z)
+        sel = np.flatnonzero(cfg_np == c)
+        G = int(sel.max()) + 1
+        l1 = torch.empty((G, enc.n), dtype=torch.float32, device="cuda")
+        b1 = torch.empty((G, enc.n), dtype=torch.uint8, device="cuda")
+        enc.channel_dev(5, 0, G, 1.0, 4.5, llr=l1, bits=b1)
+        l1, b1 = l1.cpu().numpy(), b1.cpu().numpy()
+        for g in sel:
+            o = offs[g]
+            assert bits_equal_f32(llr_np[o:o + enc.n], l1[g])
+            assert np.array_equal(bits_np[o:o + enc.n], b1[g])
+
+
+def test_mixed_decode_matches_per_code(gpu):
+    """All 114 codes in one batch (3/4B frames carry the reference encoder's
+    bytes, which fail H: they exercise non-converging decodes)."""
+    import wimax_ldpc_lib_amd as W
+    torch = gpu
+    md, cfg_np, cfg, off, offs, n_of, llr, bits = _setup(torch, 6000, 11, list(range(114)))
+    post, hard, ret, iters, synd = _mixed_decode(torch, md, cfg, off, llr)
+    post, hard = post.cpu().numpy(), hard.cpu().numpy()
+    ret, iters, synd = ret.cpu().numpy(), iters.cpu().numpy(), synd.cpu().numpy()
+    llr_np = llr.cpu().numpy()
+    assert md.invalid_count() == 0
+    for c in np.unique(cfg_np):
+        rate, z = int(c) // 19, 24 + 4 * (int(c) % 19)
+        sel = np.flatnonzero(cfg_np == c)
+        n = 24 * z
+        frames = np.stack([llr_np[offs[g]:offs[g] + n] for g in sel])
+        dec = W.ldpc_decoder(rate, z, 10, early_stop=True)
+        d_llr = torch.from_numpy(frames).cuda()
+        p1 = torch.empty_like(d_llr)
+        h1 = torch.empty((len(sel), n // 8), dtype=torch.uint8, device="cuda")
+        r1 = torch.empty(len(sel), dtype=torch.uint8, device="cuda")
+        i1 = torch.empty(len(sel), dtype=torch.uint8, device="cuda")
+        s1 = torch.empty(len(sel), dtype=torch.int16, device="cuda")
+        dec.decode_dev(d_llr, p1, h1, r1, i1, s1)
+        torch.cuda.synchronize()
+        p1, h1 = p1.cpu().numpy(), h1.cpu().numpy()
+        for q, g in enumerate(sel):
+            o = offs[g]
+            assert bits_equal_f32(post[o:o + n], p1[q]), (c, g)
+            assert np.array_equal(hard[o // 8:(o + n) // 8], h1[q]), (c, g)
+        assert np.array_equal(ret[sel], r1.cpu().numpy())
+        assert np.array_equal(iters[sel], i1.cpu().numpy())
+        assert np.array_equal(synd[sel], s1.cpu().numpy())
+
+
+def test_mixed_graph_replays(gpu):
+    """The captured graph is replayed on new contents of the same buffers;
+    every replay equals a freshly captured decode."""
+    from wimax_ldpc_lib_amd.mixed import MixedDecoder
+    torch = gpu
+    codes = [c for c in range(114) if c // 19 != 4]
+    md, cfg_np, cfg, off, offs, n_of, llr, bits = _setup(torch, 3000, 21, codes)
+    outs = []
+    for rep in range(3):
+        md.channel_dev(100 + rep, cfg, off, 1.0, 4.5, llr)
+        torch.cuda.synchronize()
+        got = [t.cpu().numpy() for t in _mixed_decode(torch, md, cfg, off, llr)]
+        fresh = MixedDecoder(max_iter=10, early_stop=True)
+        want = [t.cpu().numpy() for t in _mixed_decode(torch, fresh, cfg, off, llr)]
+        for a, b in zip(got, want):
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+        outs.append(got[1])
+    assert not np.array_equal(outs[0], outs[1])
+
+
+def test_mixed_invalid_and_disabled_codes_are_skipped(gpu):
+    torch = gpu
+    codes = [5 * 19 + 18, 1 * 19 + 18]
+    md, cfg_np, cfg, off, offs, n_of, llr, bits = _setup(torch, 100, 31, codes)
+    md.set_codes([5 * 19 + 18])
+    total = llr.numel()
+    post = torch.full((total,), 7.0, dtype=torch.float32, device="cuda")
+    ret = torch.full((100,), 9, dtype=torch.uint8, device="cuda")
+    md.decode_dev(cfg, off, llr, post, None, ret)
+    torch.cuda.synchronize()
+    skipped = cfg_np != 5 * 19 + 18
+    assert md.invalid_count() == skipped.sum()
+    r = ret.cpu().numpy()
+    assert (r[skipped] == 9).all() and (r[~skipped] <= 1).all()
+    p = post.cpu().numpy()
+    g = int(np.flatnonzero(skipped)[0])
+    assert (p[offs[g]:offs[g] + int(n_of[g])] == 7.0).all()

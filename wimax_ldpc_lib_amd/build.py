@@ -32,7 +32,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include")]
 
 HIP_SOURCES = ["decode_r0.hip", "decode_r1.hip", "decode_r2.hip", "decode_r3.hip", "decode_r4.hip",
-               "decode_r5.hip", "encoder.hip", "capi.hip", "learned.hip", "trainer.hip"]
+               "decode_r5.hip", "encoder.hip", "capi.hip", "learned.hip", "trainer.hip", "mixed.hip"]
 CXX_SOURCES = ["ldpc_classes.cpp"]
 
 

@@ -35,6 +35,15 @@ SIGNATURES = {
     "wl_compute_syndrome_host": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_uint)]),
     "wl_decoder_set_offsets": (C.c_int, [_P, _P, C.c_int]),
     "wl_decode_learned_fallback_dev": (C.c_int, [_P, _P, _SZ, C.c_int, _P, _P, _P, _P, _P, _P, _P]),
+    "wl_mixed_create": (C.c_int, [C.c_int, C.c_uint, C.POINTER(_P)]),
+    "wl_mixed_destroy": (C.c_int, [_P]),
+    "wl_mixed_set_codes": (C.c_int, [_P, _P, C.c_int]),
+    "wl_mixed_get_launch": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
+    "wl_mixed_offsets_dev": (C.c_int, [_P, _P, _SZ, _P, _P]),
+    "wl_mixed_decode_dev": (C.c_int, [_P, _P, _P, _P, _SZ, _P, _P, _P, _P, _P, _P]),
+    "wl_mixed_invalid_count": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "wl_mixed_channel_dev": (C.c_int, [_P, C.c_uint64, _P, _P, _SZ, C.c_float, C.c_float, _P, _P, _P, _P]),
     "wl_trainer_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_float, C.c_float,
                                      C.POINTER(_P)]),
     "wl_trainer_destroy": (C.c_int, [_P]),

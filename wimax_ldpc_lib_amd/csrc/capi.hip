@@ -76,27 +76,7 @@ int make_code(int rate, int z, HostCode& hc)
     return WL_OK;
 }
 
-wl::EncodeDesc make_encode_desc(const HostCode& hc)
-{
-    wl::EncodeDesc d{};
-    d.z = hc.info.z;
-    d.n = hc.info.n;
-    d.k = hc.info.k;
-    d.m = hc.info.m;
-    d.mb = hc.info.mb;
-    d.kb = 24 - hc.info.mb;
-    d.x = hc.info.x;
-    d.d = hc.info.d;
-    d.frames_per_wg = std::max(1, 256 / d.z);
-    for (int l = 0; l < hc.lt.nl; l++) {
-        d.deg[l] = (int8_t)hc.lt.deg[l];
-        for (int j = 0; j < hc.lt.deg[l]; j++) {
-            d.blk[l][j] = (int8_t)hc.lt.blk[l][j];
-            d.sh[l][j] = (int8_t)wl::lift(hc.info.rate, hc.lt.p96[l][j], d.z);
-        }
-    }
-    return d;
-}
+wl::EncodeDesc make_encode_desc(const HostCode& hc) { return wl::make_encode_desc_for(hc.info.rate, hc.info.z); }
 
 // Default frames per workgroup: the most lanes busy, then the larger group up
 // to 256 threads (the autotuner may override per (rate, z)).
@@ -158,7 +138,12 @@ struct DevBuf {
     }
 };
 
-__global__ void zero_u32_kernel(uint32_t* p) { *p = 0; }
+// fallback list header {first slot 0, count 0}
+__global__ void zero_bucket_kernel(uint32_t* p)
+{
+    p[0] = 0;
+    p[1] = 0;
+}
 
 // Frames with ret != 0 go to the fallback list; fallback[g] records it.
 __global__ void compact_failed_kernel(const uint8_t* __restrict__ ret, unsigned long long n,
@@ -343,7 +328,7 @@ int wl_decoder_set_offsets(wl_decoder* dec, const float* offsets_host, int T)
 
 static int decode_launch(wl_decoder* dec, bool learned, int max_iter, bool early_stop,
                          const float* llr, size_t nframes, float* post, uint8_t* hard, uint8_t* ret,
-                         uint8_t* iters, uint16_t* synd, const uint32_t* idx, const uint32_t* count,
+                         uint8_t* iters, uint16_t* synd, const uint32_t* idx, const uint32_t* bucket,
                          hipStream_t s)
 {
     wl::DecodeArgs a{};
@@ -355,7 +340,7 @@ static int decode_launch(wl_decoder* dec, bool learned, int max_iter, bool early
     a.synd = synd;
     a.offsets = dec->offsets.p;
     a.frame_idx = idx;
-    a.count_dev = count;
+    a.bucket = bucket;
     a.nframes = nframes;
     a.frames_per_wg = dec->frames_per_wg;
     a.max_iter = max_iter;
@@ -393,11 +378,11 @@ int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nfr
                            post, hard, ret, iters, synd, nullptr, nullptr, s);
     if (rc) return rc;
     if ((rc = dec->d_idx.ensure(nframes))) return rc;
-    if ((rc = dec->d_count.ensure(1))) return rc;
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, dec->d_count.p);
+    if ((rc = dec->d_count.ensure(2))) return rc;
+    hipLaunchKernelGGL(zero_bucket_kernel, dim3(1), dim3(1), 0, s, dec->d_count.p);
     const unsigned blocks = (unsigned)((nframes + 255) / 256);
     hipLaunchKernelGGL(compact_failed_kernel, dim3(blocks), dim3(256), 0, s, ret,
-                       (unsigned long long)nframes, dec->d_idx.p, dec->d_count.p, fallback);
+                       (unsigned long long)nframes, dec->d_idx.p, dec->d_count.p + 1, fallback);
     WL_HIP(hipGetLastError());
     return decode_launch(dec, false, fallback_iters, true, llr, nframes, post, hard, ret, iters, synd,
                          dec->d_idx.p, dec->d_count.p, s);

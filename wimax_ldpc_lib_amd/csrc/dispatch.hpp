@@ -9,6 +9,7 @@
 namespace wl {
 
 using DecodeLaunch = hipError_t (*)(const DecodeArgs&, hipStream_t);
+using DecodeOccupancy = int (*)(int frames_per_wg);
 
 constexpr int kNumVariants = 2;   // code-shape variants of the NMS kernel (tdmp.hpp VAR_*)
 
@@ -16,6 +17,8 @@ struct DecodeEntry {
     DecodeLaunch nms[kNumVariants];  // normalised min-sum, the reference's arithmetic
     DecodeLaunch learned;            // learned per-block offset min-sum (FP32)
     int fmax;                        // largest frames-per-workgroup the kernel accepts
+    DecodeOccupancy occ[kNumVariants];
+    DecodeOccupancy occ_learned;
 };
 
 // One table of kNumZ entries per rate, defined in decode_r<rate>.hip.
@@ -48,7 +51,10 @@ constexpr auto make_decode_table(std::integer_sequence<int, ZI...>)
     return Arr{{DecodeEntry{{&launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0>,
                              &launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_NOHOIST>},
                             &launch_tdmp<RATE, z_of_index(ZI), MODE_LEARNED>,
-                            max_frames_per_wg<RATE, z_of_index(ZI)>()}...}};
+                            max_frames_per_wg<RATE, z_of_index(ZI)>(),
+                            {&occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0>,
+                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_NOHOIST>},
+                            &occupancy_tdmp<RATE, z_of_index(ZI), MODE_LEARNED>}...}};
 }
 
 }  // namespace wl

@@ -18,65 +18,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "channel_dev.hpp"
 #include "encoder.hpp"
-#include "philox.hpp"
 
 namespace wl {
-
-// Shared encoder body: info bytes (already 0/1) of frame slot fs in LDS at
-// u, P_0 scratch at p0 (z bytes); lane t < z.  Writes the parity bits through
-// put(i, bit) for i in [0, m).
-template <class Put>
-__device__ __forceinline__ void encode_frame(const EncodeDesc& d, const uint8_t* u, uint8_t* p0,
-                                             int t, bool active, Put&& put)
-{
-    const int z = d.z;
-    uint8_t V[kMaxLayers];
-    uint8_t P[kMaxLayers];
-#pragma unroll
-    for (int q = 0; q < kMaxLayers; q++) {
-        uint8_t v = 0;
-        if (q < d.mb && active) {
-            for (int j = 0; j < d.deg[q]; j++) {
-                const int b = d.blk[q][j];
-                if (b >= d.kb) break;            // slots ascend by column: info blocks first
-                int p = t + d.sh[q][j];
-                p = p >= z ? p - z : p;
-                v ^= u[b * z + p];
-            }
-        }
-        V[q] = v;
-        P[q] = 0;
-    }
-    const int mb = d.mb, X = d.x;
-#pragma unroll
-    for (int q = 0; q < kMaxLayers; q++)
-        if (q == mb - 1) P[q] = V[q];
-#pragma unroll
-    for (int q = 1; q < kMaxLayers; q++)
-        if (q <= X) P[q] = P[q - 1] ^ V[q - 1];
-#pragma unroll
-    for (int q = kMaxLayers - 2; q >= 1; q--)
-        if (q <= mb - 2 && q >= X + 1) P[q] = V[q] ^ P[q + 1];
-    uint8_t vx = 0, px1 = 0, px = 0;
-#pragma unroll
-    for (int q = 0; q < kMaxLayers; q++) {
-        if (q == X) { vx = V[q]; px = P[q]; }
-        if (q == X + 1) px1 = P[q];
-    }
-    const uint8_t pz = vx ^ px1 ^ px;
-    if (active) p0[t] = pz;
-    __syncthreads();
-    if (active) {
-        int tr = t + d.d;
-        tr %= z;
-        const uint8_t corr = p0[tr];
-        put(t, pz);
-#pragma unroll
-        for (int q = 1; q < kMaxLayers; q++)
-            if (q < mb) put(q * z + t, (uint8_t)(P[q] ^ corr));
-    }
-}
 
 __global__ __launch_bounds__(256) void encode_kernel(EncodeDesc d, const uint8_t* __restrict__ info,
                                                      uint8_t* __restrict__ cw,
@@ -126,15 +71,11 @@ __global__ __launch_bounds__(256) void channel_kernel(EncodeDesc d, unsigned lon
     const bool active = fs < Feff;
     uint8_t* c = esm;                         // [F][n] codeword bits
     uint8_t* p0 = esm + (size_t)F * n;        // [F][z]
-    const uint32_t key0 = (uint32_t)seed, key1 = (uint32_t)(seed >> 32);
-    // info bits: 128 per Philox block, stream 0
-    for (int i = tid; i < Feff * ((k + 127) / 128); i += blockDim.x) {
-        const int nb = (k + 127) / 128;
+    const int nb = (k + 127) / 128;
+    for (int i = tid; i < Feff * nb; i += blockDim.x) {
         const int f = i / nb, blk = i - f * nb;
-        const unsigned long long g = frame_offset + frame0 + f;
-        const uint4 r = philox4x32_10(make_uint4((uint32_t)g, (uint32_t)(g >> 32), (uint32_t)blk, 0u),
-                                      key0, key1);
-        const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+        uint32_t w[4];
+        info_block(seed, frame_offset + frame0 + f, blk, w);
         for (int q = 0; q < 128; q++) {
             const int j = blk * 128 + q;
             if (j < k) c[(size_t)f * n + j] = (w[q >> 5] >> (q & 31)) & 1u;
@@ -145,39 +86,45 @@ __global__ __launch_bounds__(256) void channel_kernel(EncodeDesc d, unsigned lon
     encode_frame(d, c + (size_t)fsa * n, p0 + (size_t)fsa * z, t, active,
                  [&](int i, uint8_t bit) { c[(size_t)fs * n + k + i] = bit; });
     __syncthreads();
-    const float rate = (float)k / (float)n;
     for (int f = 0; f < Feff; f++) {
         const unsigned long long g = frame_offset + frame0 + f;
-        float eb = eb_lo;
-        if (eb_hi > eb_lo) {
-            const uint4 r = philox4x32_10(make_uint4((uint32_t)g, (uint32_t)(g >> 32), 0u, 2u), key0, key1);
-            eb = eb_lo + (eb_hi - eb_lo) * (float)u01(r.x);
-        }
-        const double s2 = 1.0 / (2.0 * (double)rate * pow(10.0, (double)eb / 10.0));
+        const float eb = frame_ebn0(seed, g, eb_lo, eb_hi);
+        const double s2 = frame_s2(d, eb);
         const double sig = sqrt(s2);
         if (tid == 0 && ebn0_out) ebn0_out[frame0 + f] = eb;
         for (int v2 = tid; v2 < n / 2; v2 += blockDim.x) {
-            const uint4 r = philox4x32_10(make_uint4((uint32_t)g, (uint32_t)(g >> 32), (uint32_t)v2, 1u),
-                                          key0, key1);
-            double n0, n1;
-            box_muller(r.x, r.y, n0, n1);
-            const int v = 2 * v2;
-            const uint8_t b0 = c[(size_t)f * n + v], b1 = c[(size_t)f * n + v + 1];
-            const double y0 = (b0 ? -1.0 : 1.0) + sig * n0;
-            const double y1 = (b1 ? -1.0 : 1.0) + sig * n1;
-            if (llr) {
-                // LLR = 2y/sigma^2 for bit 0 -> +1, negated to the reference convention
-                float2 o;
-                o.x = (float)(-2.0 * y0 / s2);
-                o.y = (float)(-2.0 * y1 / s2);
-                reinterpret_cast<float2*>(llr + (frame0 + f) * n)[v2] = o;
-            }
+            const uint8_t b0 = c[(size_t)f * n + 2 * v2], b1 = c[(size_t)f * n + 2 * v2 + 1];
+            if (llr) reinterpret_cast<float2*>(llr + (frame0 + f) * n)[v2] = noisy_pair(seed, g, v2, b0, b1, s2, sig);
             if (bits) {
-                bits[(frame0 + f) * n + v] = b0;
-                bits[(frame0 + f) * n + v + 1] = b1;
+                bits[(frame0 + f) * n + 2 * v2] = b0;
+                bits[(frame0 + f) * n + 2 * v2 + 1] = b1;
             }
         }
     }
+}
+
+EncodeDesc make_encode_desc_for(int rate, int z)
+{
+    const LayerTable lt = make_layer_table(rate);
+    const wl_base_info& bi = wl_base_table[rate];
+    EncodeDesc d{};
+    d.z = z;
+    d.n = 24 * z;
+    d.m = lt.nl * z;
+    d.k = d.n - d.m;
+    d.mb = lt.nl;
+    d.kb = 24 - lt.nl;
+    d.x = bi.x;
+    d.d = lift(rate, bi.d0, z);
+    d.frames_per_wg = 256 / z > 0 ? 256 / z : 1;
+    for (int l = 0; l < lt.nl; l++) {
+        d.deg[l] = (int8_t)lt.deg[l];
+        for (int j = 0; j < lt.deg[l]; j++) {
+            d.blk[l][j] = (int8_t)lt.blk[l][j];
+            d.sh[l][j] = (int8_t)lift(rate, lt.p96[l][j], z);
+        }
+    }
+    return d;
 }
 
 hipError_t launch_encode(const EncodeDesc& d, const uint8_t* info, uint8_t* cw,

@@ -17,6 +17,9 @@ struct EncodeDesc {
     int8_t sh[kMaxLayers][kMaxDeg];
 };
 
+// encoder descriptor of (rate, z); frames_per_wg = 256 / z
+EncodeDesc make_encode_desc_for(int rate, int z);
+
 hipError_t launch_encode(const EncodeDesc& d, const uint8_t* info, uint8_t* cw,
                          unsigned long long nframes, hipStream_t s);
 hipError_t launch_channel(const EncodeDesc& d, unsigned long long seed, unsigned long long frame_offset,

@@ -1,0 +1,444 @@
+// mixed.hip -- batches of frames with mixed (rate, z) in one captured launch.
+//
+// A frame's configuration id is rate * 19 + (z - 24) / 4 (114 codes).  Frame g
+// holds n_g = 24 z_g LLRs starting at element frame_off[g] of a ragged buffer.
+// One decode is a hipGraph, captured once per set of buffers and replayed:
+//   prep      zero the per-code counters and persistent tile counters
+//   hist      frames per code (workgroup histogram in LDS, then one atomic per
+//             code per workgroup); invalid ids counted separately
+//   scan      bucket start of each code (exclusive scan, one workgroup)
+//   scatter   frame ids into their code's bucket
+//   decode    one persistent kernel per code (graph siblings on 4 streams),
+//             each with its autotuned frames per workgroup and code shape,
+//             taking tiles of its bucket from an atomic counter; an empty
+//             bucket exits at once
+// Per-frame results land at the frame's own index/offset, so the output does
+// not depend on bucket order and equals a per-code decode bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "../../include/wimax_ldpc_amd.h"
+#include "channel_dev.hpp"
+#include "dispatch.hpp"
+#include "encoder.hpp"
+#include "errors.hpp"
+
+namespace wl {
+
+constexpr int kCodes = kNumRates * kNumZ;   // 114
+constexpr int kSideStreams = 4;
+
+__global__ void mixed_prep_kernel(uint32_t* counts, uint32_t* cursors, unsigned long long* tiles)
+{
+    for (int i = threadIdx.x; i <= kCodes; i += blockDim.x) {
+        counts[i] = 0;
+        cursors[i] = 0;
+    }
+    for (int i = threadIdx.x; i < kCodes; i += blockDim.x) tiles[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void mixed_hist_kernel(const uint8_t* __restrict__ cfg, unsigned long long n,
+                                                         uint32_t* __restrict__ counts)
+{
+    __shared__ uint32_t h[kCodes + 1];
+    for (int i = threadIdx.x; i <= kCodes; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (unsigned long long g = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+         g += (unsigned long long)gridDim.x * blockDim.x) {
+        const int c = cfg[g];
+        atomicAdd(&h[c < kCodes ? c : kCodes], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i <= kCodes; i += blockDim.x)
+        if (h[i]) atomicAdd(&counts[i], h[i]);
+}
+
+// buckets[c] = {start, count}
+__global__ void mixed_scan_kernel(const uint32_t* __restrict__ counts, uint32_t* __restrict__ buckets)
+{
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int c = 0; c < kCodes; c++) {
+            buckets[2 * c] = s;
+            buckets[2 * c + 1] = counts[c];
+            s += counts[c];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void mixed_scatter_kernel(const uint8_t* __restrict__ cfg, unsigned long long n,
+                                                            const uint32_t* __restrict__ buckets,
+                                                            uint32_t* __restrict__ cursors,
+                                                            uint32_t* __restrict__ idx)
+{
+    for (unsigned long long g = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+         g += (unsigned long long)gridDim.x * blockDim.x) {
+        const int c = cfg[g];
+        if (c >= kCodes) continue;
+        const uint32_t pos = atomicAdd(&cursors[c], 1u);
+        idx[buckets[2 * c] + pos] = (uint32_t)g;
+    }
+}
+
+// frame_off = exclusive prefix sum of n(cfg[g]); single pass per workgroup of
+// 1024 frames plus a carry chain over workgroups (three kernels)
+__device__ __forceinline__ unsigned long long code_n(int c) { return c < kCodes ? 24ull * (24 + 4 * (c % kNumZ)) : 0ull; }
+
+__global__ __launch_bounds__(1024) void off_block_kernel(const uint8_t* __restrict__ cfg, unsigned long long n,
+                                                         unsigned long long* __restrict__ off,
+                                                         unsigned long long* __restrict__ block_sum)
+{
+    __shared__ unsigned long long s[1024];
+    const unsigned long long g = (unsigned long long)blockIdx.x * 1024 + threadIdx.x;
+    const unsigned long long v = g < n ? code_n(cfg[g]) : 0ull;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {   // Hillis-Steele inclusive scan
+        const unsigned long long t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0ull;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (g < n) off[g] = s[threadIdx.x] - v;
+    if (threadIdx.x == 1023) block_sum[blockIdx.x] = s[1023];
+}
+
+__global__ void off_carry_kernel(unsigned long long* block_sum, unsigned long long nblocks)
+{
+    if (threadIdx.x == 0) {
+        unsigned long long acc = 0;
+        for (unsigned long long b = 0; b < nblocks; b++) {
+            const unsigned long long v = block_sum[b];
+            block_sum[b] = acc;
+            acc += v;
+        }
+        block_sum[nblocks] = acc;
+    }
+}
+
+__global__ __launch_bounds__(1024) void off_add_kernel(unsigned long long* __restrict__ off, unsigned long long n,
+                                                       const unsigned long long* __restrict__ block_sum)
+{
+    const unsigned long long g = (unsigned long long)blockIdx.x * 1024 + threadIdx.x;
+    if (g < n) off[g] += block_sum[blockIdx.x];
+}
+
+// Mixed channel: one frame per workgroup, the frame's own code; frame g of
+// code c equals frame g of wl_channel_dev on an encoder of code c.
+__global__ __launch_bounds__(128) void mixed_channel_kernel(const EncodeDesc* __restrict__ descs,
+                                                            const uint8_t* __restrict__ cfg,
+                                                            const unsigned long long* __restrict__ off,
+                                                            unsigned long long nframes,
+                                                            unsigned long long seed, float eb_lo, float eb_hi,
+                                                            float* __restrict__ llr, uint8_t* __restrict__ bits,
+                                                            float* __restrict__ ebn0_out)
+{
+    __shared__ uint8_t c[2304 + 96];
+    for (unsigned long long g = blockIdx.x; g < nframes; g += gridDim.x) {
+        const int code = cfg[g];
+        if (code >= kCodes) continue;   // uniform per workgroup
+        generate_frame(descs[code], seed, g, eb_lo, eb_hi, c, llr + off[g], bits ? bits + off[g] : nullptr,
+                       ebn0_out ? ebn0_out + g : nullptr);
+        __syncthreads();
+    }
+}
+
+}  // namespace wl
+
+using namespace wl;
+
+struct wl_mixed {
+    int max_iter = 10;
+    unsigned flags = 0;
+    int num_cu = 0;
+    const DecodeEntry* entries[kCodes] = {};
+    int F[kCodes] = {}, variant[kCodes] = {}, grid[kCodes] = {}, occ[kCodes] = {};
+    bool enabled[kCodes] = {};
+    uint32_t *counts = nullptr, *cursors = nullptr, *buckets = nullptr, *idx = nullptr;
+    unsigned long long* tiles = nullptr;
+    size_t idx_cap = 0;
+    unsigned long long* block_sum = nullptr;
+    size_t bs_cap = 0;
+    EncodeDesc* descs = nullptr;
+    hipStream_t side[kSideStreams] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kSideStreams] = {};
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    // buffers the captured graph refers to
+    const void* key[9] = {};
+    size_t key_n = 0;
+};
+
+namespace {
+
+int mfail(int code, const char* msg)
+{
+    set_error_text(msg);
+    return code;
+}
+
+int hipf(hipError_t e, const char* what)
+{
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    set_error_text(buf);
+    return WL_EDEVICE;
+}
+
+void drop_graph(wl_mixed* m)
+{
+    if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    m->gexec = nullptr;
+    m->graph = nullptr;
+}
+
+int ensure_idx(wl_mixed* m, size_t n)
+{
+    if (n <= m->idx_cap) return WL_OK;
+    if (m->idx) (void)hipFree(m->idx);
+    m->idx = nullptr;
+    m->idx_cap = 0;
+    hipError_t e = hipMalloc((void**)&m->idx, std::max<size_t>(n, 1) * sizeof(uint32_t));
+    if (e != hipSuccess) return hipf(e, "hipMalloc");
+    m->idx_cap = n;
+    drop_graph(m);
+    return WL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wl_mixed_create(int max_iter, unsigned flags, wl_mixed** out)
+{
+    if (!out) return mfail(WL_EINVAL, "null output");
+    *out = nullptr;
+    if (max_iter < 0 || max_iter > 255) return mfail(WL_EINVAL, "max_iter out of range 0..255");
+    int dev = 0;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) return hipf(e, "device query");
+    wl_mixed* m = new (std::nothrow) wl_mixed();
+    if (!m) return mfail(WL_ENOMEM, "out of host memory");
+    m->max_iter = max_iter;
+    m->flags = flags;
+    m->num_cu = prop.multiProcessorCount;
+    std::vector<EncodeDesc> descs(kCodes);
+    for (int c = 0; c < kCodes; c++) {
+        const int rate = c / kNumZ, z = z_of_index(c % kNumZ);
+        m->entries[c] = decode_entry(rate, z);
+        int f = 0, v = 0;
+        if (wl_tuned_launch(rate, z, &f, &v) < 0) {
+            delete m;
+            return WL_EINVAL;
+        }
+        m->F[c] = f;
+        m->variant[c] = v;
+        const int occ = m->entries[c]->occ[v](f);
+        m->occ[c] = occ;
+        m->grid[c] = std::max(1, occ) * m->num_cu;
+        m->enabled[c] = true;
+        descs[c] = make_encode_desc_for(rate, z);
+    }
+    e = hipMalloc((void**)&m->counts, (kCodes + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->cursors, (kCodes + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->buckets, 2 * kCodes * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->tiles, kCodes * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->descs, kCodes * sizeof(EncodeDesc));
+    if (e == hipSuccess) e = hipMemcpy(m->descs, descs.data(), kCodes * sizeof(EncodeDesc), hipMemcpyHostToDevice);
+    for (int i = 0; i < kSideStreams && e == hipSuccess; i++) {
+        e = hipStreamCreateWithFlags(&m->side[i], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev_join[i], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->cap_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        wl_mixed_destroy(m);
+        return hipf(e, "mixed decoder allocation");
+    }
+    *out = m;
+    return WL_OK;
+}
+
+int wl_mixed_destroy(wl_mixed* m)
+{
+    if (!m) return WL_OK;
+    drop_graph(m);
+    void* ps[] = {m->counts, m->cursors, m->buckets, m->idx, m->tiles, m->block_sum, m->descs};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    for (int i = 0; i < kSideStreams; i++) {
+        if (m->side[i]) (void)hipStreamDestroy(m->side[i]);
+        if (m->ev_join[i]) (void)hipEventDestroy(m->ev_join[i]);
+    }
+    if (m->ev_fork) (void)hipEventDestroy(m->ev_fork);
+    if (m->cap_stream) (void)hipStreamDestroy(m->cap_stream);
+    delete m;
+    return WL_OK;
+}
+
+int wl_mixed_set_codes(wl_mixed* m, const int* codes, int ncodes)
+{
+    if (!m) return mfail(WL_EINVAL, "null mixed decoder");
+    if (ncodes < 0 || (ncodes > 0 && !codes)) return mfail(WL_EINVAL, "bad code list");
+    bool en[kCodes] = {};
+    for (int i = 0; i < ncodes; i++) {
+        if (codes[i] < 0 || codes[i] >= kCodes) return mfail(WL_EINVAL, "code id out of range 0..113");
+        en[codes[i]] = true;
+    }
+    for (int c = 0; c < kCodes; c++) m->enabled[c] = ncodes == 0 ? true : en[c];
+    drop_graph(m);
+    return WL_OK;
+}
+
+int wl_mixed_get_launch(const wl_mixed* m, int rate, int z, int* frames_per_wg, int* variant, int* grid)
+{
+    if (!m || !frames_per_wg || !variant || !grid) return mfail(WL_EINVAL, "null argument");
+    if (rate < 0 || rate >= kNumRates || !valid_z(z)) return mfail(WL_EINVAL, "invalid rate/z");
+    const int c = rate * kNumZ + index_of_z(z);
+    *frames_per_wg = m->F[c];
+    *variant = m->variant[c];
+    *grid = m->grid[c];
+    return WL_OK;
+}
+
+int wl_mixed_offsets_dev(wl_mixed* m, const uint8_t* cfg, size_t nframes, uint64_t* frame_off, void* stream)
+{
+    if (!m || !cfg || !frame_off) return mfail(WL_EINVAL, "null argument");
+    if (nframes == 0) return WL_OK;
+    const size_t nb = (nframes + 1023) / 1024;
+    if (nb + 1 > m->bs_cap) {
+        if (m->block_sum) (void)hipFree(m->block_sum);
+        m->block_sum = nullptr;
+        m->bs_cap = 0;
+        hipError_t e = hipMalloc((void**)&m->block_sum, (nb + 1) * sizeof(unsigned long long));
+        if (e != hipSuccess) return hipf(e, "hipMalloc");
+        m->bs_cap = nb + 1;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    auto* off = reinterpret_cast<unsigned long long*>(frame_off);
+    hipLaunchKernelGGL(off_block_kernel, dim3((unsigned)nb), dim3(1024), 0, s, cfg, (unsigned long long)nframes,
+                       off, m->block_sum);
+    hipLaunchKernelGGL(off_carry_kernel, dim3(1), dim3(64), 0, s, m->block_sum, (unsigned long long)nb);
+    hipLaunchKernelGGL(off_add_kernel, dim3((unsigned)nb), dim3(1024), 0, s, off, (unsigned long long)nframes,
+                       m->block_sum);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WL_OK : hipf(e, "offset kernels");
+}
+
+int wl_mixed_channel_dev(wl_mixed* m, uint64_t seed, const uint8_t* cfg, const uint64_t* frame_off,
+                         size_t nframes, float ebn0_lo, float ebn0_hi, float* llr, uint8_t* bits, float* ebn0,
+                         void* stream)
+{
+    if (!m || !cfg || !frame_off || !llr) return mfail(WL_EINVAL, "null argument");
+    if (nframes == 0) return WL_OK;
+    const unsigned grid = (unsigned)std::min<size_t>(nframes, (size_t)m->num_cu * 16);
+    hipLaunchKernelGGL(mixed_channel_kernel, dim3(grid), dim3(128), 0, (hipStream_t)stream, m->descs, cfg,
+                       reinterpret_cast<const unsigned long long*>(frame_off), (unsigned long long)nframes,
+                       (unsigned long long)seed, ebn0_lo, ebn0_hi, llr, bits, ebn0);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WL_OK : hipf(e, "mixed channel kernel");
+}
+
+static int capture_decode(wl_mixed* m, const uint8_t* cfg, const uint64_t* frame_off, const float* llr,
+                          size_t nframes, float* post, uint8_t* hard, uint8_t* ret, uint8_t* iters,
+                          uint16_t* synd)
+{
+    hipStream_t s = m->cap_stream;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hipf(e, "begin capture");
+    const unsigned long long n = nframes;
+    const unsigned blocks = (unsigned)std::min<unsigned long long>((n + 255) / 256, (unsigned long long)m->num_cu * 8);
+    hipLaunchKernelGGL(mixed_prep_kernel, dim3(1), dim3(128), 0, s, m->counts, m->cursors, m->tiles);
+    hipLaunchKernelGGL(mixed_hist_kernel, dim3(blocks), dim3(256), 0, s, cfg, n, m->counts);
+    hipLaunchKernelGGL(mixed_scan_kernel, dim3(1), dim3(64), 0, s, m->counts, m->buckets);
+    hipLaunchKernelGGL(mixed_scatter_kernel, dim3(blocks), dim3(256), 0, s, cfg, n, m->buckets, m->cursors, m->idx);
+    e = hipEventRecord(m->ev_fork, s);
+    for (int i = 0; i < kSideStreams && e == hipSuccess; i++) e = hipStreamWaitEvent(m->side[i], m->ev_fork, 0);
+    int launched = 0;
+    for (int c = 0; c < kCodes && e == hipSuccess; c++) {
+        if (!m->enabled[c]) continue;
+        DecodeArgs a{};
+        a.llr = llr;
+        a.post = post;
+        a.hard = hard;
+        a.ret = ret;
+        a.iters = iters;
+        a.synd = synd;
+        a.frame_idx = m->idx;
+        a.bucket = m->buckets + 2 * c;
+        a.frame_off = reinterpret_cast<const unsigned long long*>(frame_off);
+        a.tile_counter = m->tiles + c;
+        a.persistent_grid = (int)std::min<unsigned long long>((unsigned long long)m->grid[c],
+                                                              (n + m->F[c] - 1) / m->F[c]);
+        a.nframes = n;
+        a.frames_per_wg = m->F[c];
+        a.max_iter = m->max_iter;
+        a.early_stop = (m->flags & WL_FLAG_EARLY_STOP) ? 1 : 0;
+        e = m->entries[c]->nms[m->variant[c]](a, m->side[launched % kSideStreams]);
+        launched++;
+    }
+    for (int i = 0; i < kSideStreams && e == hipSuccess; i++) {
+        e = hipEventRecord(m->ev_join[i], m->side[i]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, m->ev_join[i], 0);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hipf(e != hipSuccess ? e : e2, "mixed decode capture");
+    }
+    e = hipGraphInstantiate(&m->gexec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hipf(e, "graph instantiate");
+    }
+    m->graph = g;
+    return WL_OK;
+}
+
+int wl_mixed_decode_dev(wl_mixed* m, const uint8_t* cfg, const uint64_t* frame_off, const float* llr,
+                        size_t nframes, float* post, uint8_t* hard, uint8_t* ret, uint8_t* iters, uint16_t* synd,
+                        void* stream)
+{
+    if (!m) return mfail(WL_EINVAL, "null mixed decoder");
+    if (nframes == 0) return WL_OK;
+    if (!cfg || !frame_off || !llr) return mfail(WL_EINVAL, "cfg, frame_off and llr are required");
+    if (nframes > 0xffffffffull) return mfail(WL_EINVAL, "nframes too large");
+    int rc = ensure_idx(m, nframes);
+    if (rc) return rc;
+    const void* key[9] = {cfg, frame_off, llr, post, hard, ret, iters, synd, nullptr};
+    if (!m->gexec || m->key_n != nframes || memcmp(key, m->key, sizeof key) != 0) {
+        drop_graph(m);
+        rc = capture_decode(m, cfg, frame_off, llr, nframes, post, hard, ret, iters, synd);
+        if (rc) return rc;
+        memcpy(m->key, key, sizeof key);
+        m->key_n = nframes;
+    }
+    hipError_t e = hipGraphLaunch(m->gexec, (hipStream_t)stream);
+    return e == hipSuccess ? WL_OK : hipf(e, "graph launch");
+}
+
+int wl_mixed_invalid_count(wl_mixed* m, uint64_t* out)
+{
+    if (!m || !out) return mfail(WL_EINVAL, "null argument");
+    uint32_t v[kCodes + 1] = {};
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(v, m->counts, sizeof v, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hipf(e, "read frame counts");
+    uint64_t skipped = v[kCodes];
+    for (int c = 0; c < kCodes; c++)
+        if (!m->enabled[c]) skipped += v[c];
+    *out = skipped;
+    return WL_OK;
+}
+
+}  // extern "C"

@@ -37,8 +37,12 @@ struct DecodeArgs {
     uint8_t* iters;            // [nframes] iterations run (nullable)
     uint16_t* synd;            // [nframes] failing parity checks of the output (nullable)
     const float* offsets;      // learned mode: [T][NL][C] per-block offsets
-    const uint32_t* frame_idx; // nullable: slot i decodes frame frame_idx[i] (in and out)
-    const uint32_t* count_dev; // with frame_idx: number of valid slots, read on the device
+    const uint32_t* frame_idx; // nullable: slot i decodes frame frame_idx[bucket[0] + i]
+    const uint32_t* bucket;    // nullable: {first slot, slot count}, read on the device
+    const unsigned long long* frame_off;  // nullable: element offset of frame g in llr/post
+                                          // (hard: frame_off[g] / 8 bytes); default g * n
+    unsigned long long* tile_counter;     // nullable: persistent mode, tiles taken from this counter
+    int persistent_grid;                  // persistent mode: workgroups launched (<= tiles)
     unsigned long long nframes;
     int frames_per_wg;         // F: frames decoded together by one workgroup
     int max_iter;
@@ -49,6 +53,7 @@ struct DecodeArgs {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kLdsPad = 512;   // per-frame counters; also keeps wrapped row bases >= 0
+constexpr int kTileSlot = 504; // persistent mode: the tile index broadcast, inside the pad
 constexpr int kMaxThreads = 512;
 constexpr int kMaxLdsBytes = 65536;
 
@@ -70,46 +75,45 @@ constexpr int max_frames_per_wg()
     return f;
 }
 
-template <int RATE, int Z, int MODE, int VAR = 0>
-__global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
+// One tile: frames [frame0, frame0 + Feff) of the (possibly indirect) frame
+// list, decoded by one workgroup.  All threads of the workgroup call it.
+template <int RATE, int Z, int MODE, int VAR>
+__device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* smem, unsigned long long frame0,
+                                          int Feff)
 {
     using CD = Code<RATE, Z>;
     constexpr int NL = CD::NL;
     constexpr int N = CD::N;
     constexpr int C = CD::C;
 
-    extern __shared__ __align__(16) unsigned char smem[];
     uint32_t* cnt = reinterpret_cast<uint32_t*>(smem);
     float* rxall = reinterpret_cast<float*>(smem + kLdsPad);
 
     const int tid = threadIdx.x;
     const int F = a.frames_per_wg;
-    const unsigned long long frame0 = (unsigned long long)blockIdx.x * (unsigned long long)F;
-    unsigned long long nfr = a.nframes;
-    if (a.frame_idx) {
-        const unsigned long long c = *a.count_dev;
-        nfr = c < nfr ? c : nfr;
-    }
-    if (frame0 >= nfr) return;   // whole workgroup: no barrier has been reached yet
-    const unsigned long long left = nfr - frame0;
-    const int Feff = left < (unsigned long long)F ? (int)left : F;
     const int f = tid / Z;
     const int r = tid - f * Z;
     const bool active = f < Feff;
-    // global frame of slot s
+    const unsigned long long slot0 = a.bucket ? (unsigned long long)a.bucket[0] : 0ull;
+    // global frame of slot s, and its element offset
     auto gframe = [&](int s) -> unsigned long long {
-        return a.frame_idx ? (unsigned long long)a.frame_idx[frame0 + s] : frame0 + s;
+        return a.frame_idx ? (unsigned long long)a.frame_idx[slot0 + frame0 + s] : frame0 + s;
     };
+    auto goff = [&](int s) -> unsigned long long {
+        const unsigned long long g = gframe(s);
+        return a.frame_off ? a.frame_off[g] : g * N;
+    };
+    const bool contiguous = !a.frame_idx && !a.frame_off;
 
     if (tid < F) cnt[tid] = 0;
-    if (!a.frame_idx) {
+    if (contiguous) {
         const f32x4* src = reinterpret_cast<const f32x4*>(a.llr + frame0 * N);
         f32x4* dst = reinterpret_cast<f32x4*>(rxall);
         const int total4 = Feff * (N / 4);
         for (int i = tid; i < total4; i += blockDim.x) dst[i] = __builtin_nontemporal_load(src + i);
     } else {
         for (int s = 0; s < Feff; s++) {
-            const f32x4* src = reinterpret_cast<const f32x4*>(a.llr + gframe(s) * N);
+            const f32x4* src = reinterpret_cast<const f32x4*>(a.llr + goff(s));
             f32x4* dst = reinterpret_cast<f32x4*>(rxall + s * N);
             for (int i = tid; i < N / 4; i += blockDim.x) dst[i] = src[i];
         }
@@ -290,7 +294,7 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
 
     if (a.post) {
         for (int s = 0; s < Feff; s++) {
-            f32x4* dst = reinterpret_cast<f32x4*>(a.post + gframe(s) * N);
+            f32x4* dst = reinterpret_cast<f32x4*>(a.post + goff(s));
             const f32x4* src = reinterpret_cast<const f32x4*>(rxall + s * N);
             for (int i = tid; i < N / 4; i += blockDim.x) __builtin_nontemporal_store(src[i], dst + i);
         }
@@ -306,9 +310,54 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
             const bool bit = v < N && rxall[s * N + v] >= 0.0f;
             const unsigned long long m = __ballot(bit);
             const int byte = (c0 >> 3) + lane;
-            if (lane < 8 && byte < N / 8) a.hard[gframe(s) * (N / 8) + byte] = (uint8_t)(m >> (8 * lane));
+            if (lane < 8 && byte < N / 8) a.hard[goff(s) / 8 + byte] = (uint8_t)(m >> (8 * lane));
         }
     }
+}
+
+template <int RATE, int Z, int MODE, int VAR = 0>
+__global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int F = a.frames_per_wg;
+    unsigned long long nfr = a.nframes;
+    if (a.bucket) {
+        const unsigned long long c = a.bucket[1];
+        nfr = c < nfr ? c : nfr;
+    }
+    if (!a.tile_counter) {
+        const unsigned long long frame0 = (unsigned long long)blockIdx.x * (unsigned long long)F;
+        if (frame0 >= nfr) return;   // whole workgroup, before any barrier
+        const unsigned long long left = nfr - frame0;
+        tdmp_tile<RATE, Z, MODE, VAR>(a, smem, frame0, left < (unsigned long long)F ? (int)left : F);
+        return;
+    }
+    // persistent: take tiles until the list is exhausted (every workgroup
+    // reaches the exit: the counter only grows)
+    unsigned long long* slot = reinterpret_cast<unsigned long long*>(smem + kTileSlot);
+    for (;;) {
+        if (threadIdx.x == 0) *slot = atomicAdd(a.tile_counter, 1ull);
+        __syncthreads();
+        const unsigned long long frame0 = *slot * (unsigned long long)F;
+        __syncthreads();
+        if (frame0 >= nfr) break;
+        const unsigned long long left = nfr - frame0;
+        tdmp_tile<RATE, Z, MODE, VAR>(a, smem, frame0, left < (unsigned long long)F ? (int)left : F);
+        __syncthreads();
+    }
+}
+
+// Workgroups of this kernel resident per CU at frames_per_wg F (0 on error).
+template <int RATE, int Z, int MODE, int VAR = 0>
+int occupancy_tdmp(int F)
+{
+    const int threads = ((F * Z + 63) / 64) * 64;
+    const size_t lds = kLdsPad + (size_t)F * Code<RATE, Z>::N * sizeof(float);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tdmp_kernel<RATE, Z, MODE, VAR>, threads, lds) !=
+        hipSuccess)
+        return 0;
+    return nb;
 }
 
 template <int RATE, int Z, int MODE, int VAR = 0>
@@ -319,7 +368,9 @@ hipError_t launch_tdmp(const DecodeArgs& a, hipStream_t stream)
     if (a.nframes == 0) return hipSuccess;
     const int threads = ((F * Z + 63) / 64) * 64;
     const size_t lds = kLdsPad + (size_t)F * Code<RATE, Z>::N * sizeof(float);
-    const unsigned long long grid = (a.nframes + F - 1) / F;
+    unsigned long long grid = (a.nframes + F - 1) / F;
+    if (a.tile_counter && a.persistent_grid > 0 && (unsigned long long)a.persistent_grid < grid)
+        grid = (unsigned long long)a.persistent_grid;
     if (grid > 0x7fffffffull) return hipErrorInvalidValue;
     hipLaunchKernelGGL((tdmp_kernel<RATE, Z, MODE, VAR>), dim3((unsigned)grid), dim3(threads), lds, stream, a);
     return hipGetLastError();
