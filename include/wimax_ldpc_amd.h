@@ -140,7 +140,10 @@ int wl_decoder_set_offsets(wl_decoder* dec, const float* offsets_host, int T);
 /* Learned decode with fallback: frames whose hard decisions fail the
  * syndrome after the learned iterations are decoded again from their input
  * LLRs with normalised min-sum, fallback_iters iterations, early stop.
- * fallback (nullable) [nframes] gets 1 for re-decoded frames. */
+ * fallback (nullable) [nframes] gets 1 for re-decoded frames.  For z = 24 and
+ * z = 96 this is one persistent kernel that re-queues failed frames in the
+ * workgroup and decodes them itself; other z use a learned pass, a device-side
+ * compaction of the failures and a normalised min-sum pass over them. */
 int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nframes,
                                    int fallback_iters, float* post, uint8_t* hard,
                                    uint8_t* ret, uint8_t* iters, uint16_t* synd,
@@ -170,6 +173,19 @@ int wl_mixed_offsets_dev(wl_mixed* m, const uint8_t* cfg, size_t nframes, uint64
 int wl_mixed_decode_dev(wl_mixed* m, const uint8_t* cfg, const uint64_t* frame_off, const float* llr,
                         size_t nframes, float* post, uint8_t* hard, uint8_t* ret, uint8_t* iters,
                         uint16_t* synd, void* stream);
+/* Learned offsets [T][mb][c] for every code of `rate` (NULL clears). */
+int wl_mixed_set_offsets(wl_mixed* m, int rate, const float* offsets_host, int T);
+/* Learned decoding with re-queue: codes whose rate has offsets (and whose z is
+ * 24 or 96) are decoded by one persistent kernel per code that runs the
+ * learned decoder (max_iter iterations, the early-stop flag of create()) and
+ * re-queues each frame whose syndrome still fails to normalised min-sum with
+ * fallback_iters iterations and early stop, from the frame's input LLRs, in
+ * the same kernel; fallback [nframes] (nullable) gets 1 for such frames.
+ * Other codes are decoded with normalised min-sum as wl_mixed_decode_dev. */
+int wl_mixed_decode_learned_fallback_dev(wl_mixed* m, const uint8_t* cfg, const uint64_t* frame_off,
+                                         const float* llr, size_t nframes, int fallback_iters, float* post,
+                                         uint8_t* hard, uint8_t* ret, uint8_t* iters, uint16_t* synd,
+                                         uint8_t* fallback, void* stream);
 /* Frames skipped by the last decode for an invalid or disabled code id
  * (synchronizes the device). */
 int wl_mixed_invalid_count(wl_mixed* m, uint64_t* out);

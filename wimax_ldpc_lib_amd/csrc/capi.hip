@@ -374,8 +374,39 @@ int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nfr
     if (fallback_iters < 0 || fallback_iters > 255) return fail(WL_EINVAL, "fallback_iters out of range");
     if (nframes > 0xffffffffull) return fail(WL_EINVAL, "nframes %zu too large", nframes);
     hipStream_t s = (hipStream_t)stream;
-    int rc = decode_launch(dec, true, dec->max_iter, (dec->flags & WL_FLAG_EARLY_STOP) != 0, llr, nframes,
-                           post, hard, ret, iters, synd, nullptr, nullptr, s);
+    int rc;
+    if (dec->entry->learned_fb) {
+        // one persistent kernel: learned tiles, failures re-queued in LDS and
+        // decoded again with normalised min-sum by the same workgroup
+        if ((rc = dec->d_count.ensure(2))) return rc;
+        hipLaunchKernelGGL(zero_bucket_kernel, dim3(1), dim3(1), 0, s, dec->d_count.p);
+        wl::DecodeArgs a{};
+        a.llr = llr;
+        a.post = post;
+        a.hard = hard;
+        a.ret = ret;
+        a.iters = iters;
+        a.synd = synd;
+        a.offsets = dec->offsets.p;
+        a.tile_counter = reinterpret_cast<unsigned long long*>(dec->d_count.p);
+        a.nframes = nframes;
+        a.frames_per_wg = dec->frames_per_wg;
+        a.max_iter = dec->max_iter;
+        a.early_stop = (dec->flags & WL_FLAG_EARLY_STOP) ? 1 : 0;
+        a.n_offset_iters = dec->T;
+        a.fallback_iters = fallback_iters;
+        a.fallback = fallback;
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess) {
+            const int occ = dec->entry->occ_learned_fb(dec->frames_per_wg);
+            a.persistent_grid = (occ > 0 ? occ : 1) * ncu;
+        }
+        hipError_t e = dec->entry->learned_fb(a, s);
+        if (e != hipSuccess) return hip_fail(e, "learned fallback kernel launch");
+        return WL_OK;
+    }
+    rc = decode_launch(dec, true, dec->max_iter, (dec->flags & WL_FLAG_EARLY_STOP) != 0, llr, nframes, post, hard,
+                       ret, iters, synd, nullptr, nullptr, s);
     if (rc) return rc;
     if ((rc = dec->d_idx.ensure(nframes))) return rc;
     if ((rc = dec->d_count.ensure(2))) return rc;

@@ -19,7 +19,24 @@ struct DecodeEntry {
     int fmax;                        // largest frames-per-workgroup the kernel accepts
     DecodeOccupancy occ[kNumVariants];
     DecodeOccupancy occ_learned;
+    DecodeLaunch learned_fb;         // learned + in-kernel normalised min-sum re-queue
+                                     // (persistent only; built for z = 24 and 96, else null)
+    DecodeOccupancy occ_learned_fb;
 };
+
+template <int RATE, int Z>
+constexpr DecodeLaunch fb_launcher()
+{
+    if constexpr (Z == 24 || Z == 96) return &launch_tdmp<RATE, Z, MODE_LEARNED_FB, VAR_NOHOIST>;
+    else return nullptr;
+}
+
+template <int RATE, int Z>
+constexpr DecodeOccupancy fb_occupancy()
+{
+    if constexpr (Z == 24 || Z == 96) return &occupancy_tdmp<RATE, Z, MODE_LEARNED_FB, VAR_NOHOIST>;
+    else return nullptr;
+}
 
 // One table of kNumZ entries per rate, defined in decode_r<rate>.hip.
 const DecodeEntry* decode_table_r0();
@@ -54,7 +71,9 @@ constexpr auto make_decode_table(std::integer_sequence<int, ZI...>)
                             max_frames_per_wg<RATE, z_of_index(ZI)>(),
                             {&occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0>,
                              &occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_NOHOIST>},
-                            &occupancy_tdmp<RATE, z_of_index(ZI), MODE_LEARNED>}...}};
+                            &occupancy_tdmp<RATE, z_of_index(ZI), MODE_LEARNED>,
+                            fb_launcher<RATE, z_of_index(ZI)>(),
+                            fb_occupancy<RATE, z_of_index(ZI)>()}...}};
 }
 
 }  // namespace wl

@@ -168,11 +168,16 @@ struct wl_mixed {
     hipStream_t side[kSideStreams] = {};
     hipEvent_t ev_fork = nullptr, ev_join[kSideStreams] = {};
     hipStream_t cap_stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    // buffers the captured graph refers to
-    const void* key[9] = {};
-    size_t key_n = 0;
+    // captured graphs: [0] plain decode, [1] learned + fallback
+    hipGraph_t graph[2] = {};
+    hipGraphExec_t gexec[2] = {};
+    const void* key[2][10] = {};   // buffers each graph refers to
+    size_t key_n[2] = {};
+    int key_it[2] = {};
+    // learned offsets per rate: [T][mb][c]
+    float* offsets[kNumRates] = {};
+    int T[kNumRates] = {};
+    int ncu = 0;
 };
 
 namespace {
@@ -193,10 +198,12 @@ int hipf(hipError_t e, const char* what)
 
 void drop_graph(wl_mixed* m)
 {
-    if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
-    if (m->graph) (void)hipGraphDestroy(m->graph);
-    m->gexec = nullptr;
-    m->graph = nullptr;
+    for (int i = 0; i < 2; i++) {
+        if (m->gexec[i]) (void)hipGraphExecDestroy(m->gexec[i]);
+        if (m->graph[i]) (void)hipGraphDestroy(m->graph[i]);
+        m->gexec[i] = nullptr;
+        m->graph[i] = nullptr;
+    }
 }
 
 int ensure_idx(wl_mixed* m, size_t n)
@@ -275,6 +282,8 @@ int wl_mixed_destroy(wl_mixed* m)
     void* ps[] = {m->counts, m->cursors, m->buckets, m->idx, m->tiles, m->block_sum, m->descs};
     for (void* p : ps)
         if (p) (void)hipFree(p);
+    for (int r = 0; r < kNumRates; r++)
+        if (m->offsets[r]) (void)hipFree(m->offsets[r]);
     for (int i = 0; i < kSideStreams; i++) {
         if (m->side[i]) (void)hipStreamDestroy(m->side[i]);
         if (m->ev_join[i]) (void)hipEventDestroy(m->ev_join[i]);
@@ -348,9 +357,9 @@ int wl_mixed_channel_dev(wl_mixed* m, uint64_t seed, const uint8_t* cfg, const u
     return e == hipSuccess ? WL_OK : hipf(e, "mixed channel kernel");
 }
 
-static int capture_decode(wl_mixed* m, const uint8_t* cfg, const uint64_t* frame_off, const float* llr,
-                          size_t nframes, float* post, uint8_t* hard, uint8_t* ret, uint8_t* iters,
-                          uint16_t* synd)
+static int capture_decode(wl_mixed* m, int learned, int fallback_iters, const uint8_t* cfg,
+                          const uint64_t* frame_off, const float* llr, size_t nframes, float* post, uint8_t* hard,
+                          uint8_t* ret, uint8_t* iters, uint16_t* synd, uint8_t* fallback)
 {
     hipStream_t s = m->cap_stream;
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
@@ -383,7 +392,20 @@ static int capture_decode(wl_mixed* m, const uint8_t* cfg, const uint64_t* frame
         a.frames_per_wg = m->F[c];
         a.max_iter = m->max_iter;
         a.early_stop = (m->flags & WL_FLAG_EARLY_STOP) ? 1 : 0;
-        e = m->entries[c]->nms[m->variant[c]](a, m->side[launched % kSideStreams]);
+        const int rate = c / kNumZ;
+        hipStream_t ss = m->side[launched % kSideStreams];
+        if (learned && m->offsets[rate] && m->entries[c]->learned_fb) {
+            a.offsets = m->offsets[rate];
+            a.n_offset_iters = m->T[rate];
+            a.fallback_iters = fallback_iters;
+            a.fallback = fallback;
+            const int occ = m->entries[c]->occ_learned_fb(m->F[c]);
+            a.persistent_grid = (int)std::min<unsigned long long>((unsigned long long)std::max(1, occ) * m->num_cu,
+                                                                  (n + m->F[c] - 1) / m->F[c]);
+            e = m->entries[c]->learned_fb(a, ss);
+        } else {
+            e = m->entries[c]->nms[m->variant[c]](a, ss);
+        }
         launched++;
     }
     for (int i = 0; i < kSideStreams && e == hipSuccess; i++) {
@@ -396,13 +418,37 @@ static int capture_decode(wl_mixed* m, const uint8_t* cfg, const uint64_t* frame
         if (g) (void)hipGraphDestroy(g);
         return hipf(e != hipSuccess ? e : e2, "mixed decode capture");
     }
-    e = hipGraphInstantiate(&m->gexec, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&m->gexec[learned], g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
         (void)hipGraphDestroy(g);
         return hipf(e, "graph instantiate");
     }
-    m->graph = g;
+    m->graph[learned] = g;
     return WL_OK;
+}
+
+static int run_graph(wl_mixed* m, int learned, int fallback_iters, const uint8_t* cfg, const uint64_t* frame_off,
+                     const float* llr, size_t nframes, float* post, uint8_t* hard, uint8_t* ret, uint8_t* iters,
+                     uint16_t* synd, uint8_t* fallback, hipStream_t stream)
+{
+    int rc = ensure_idx(m, nframes);
+    if (rc) return rc;
+    const void* key[10] = {cfg, frame_off, llr, post, hard, ret, iters, synd, fallback, nullptr};
+    if (!m->gexec[learned] || m->key_n[learned] != nframes || m->key_it[learned] != fallback_iters ||
+        memcmp(key, m->key[learned], sizeof key) != 0) {
+        if (m->gexec[learned]) (void)hipGraphExecDestroy(m->gexec[learned]);
+        if (m->graph[learned]) (void)hipGraphDestroy(m->graph[learned]);
+        m->gexec[learned] = nullptr;
+        m->graph[learned] = nullptr;
+        rc = capture_decode(m, learned, fallback_iters, cfg, frame_off, llr, nframes, post, hard, ret, iters, synd,
+                            fallback);
+        if (rc) return rc;
+        memcpy(m->key[learned], key, sizeof key);
+        m->key_n[learned] = nframes;
+        m->key_it[learned] = fallback_iters;
+    }
+    hipError_t e = hipGraphLaunch(m->gexec[learned], stream);
+    return e == hipSuccess ? WL_OK : hipf(e, "graph launch");
 }
 
 int wl_mixed_decode_dev(wl_mixed* m, const uint8_t* cfg, const uint64_t* frame_off, const float* llr,
@@ -413,18 +459,41 @@ int wl_mixed_decode_dev(wl_mixed* m, const uint8_t* cfg, const uint64_t* frame_o
     if (nframes == 0) return WL_OK;
     if (!cfg || !frame_off || !llr) return mfail(WL_EINVAL, "cfg, frame_off and llr are required");
     if (nframes > 0xffffffffull) return mfail(WL_EINVAL, "nframes too large");
-    int rc = ensure_idx(m, nframes);
-    if (rc) return rc;
-    const void* key[9] = {cfg, frame_off, llr, post, hard, ret, iters, synd, nullptr};
-    if (!m->gexec || m->key_n != nframes || memcmp(key, m->key, sizeof key) != 0) {
-        drop_graph(m);
-        rc = capture_decode(m, cfg, frame_off, llr, nframes, post, hard, ret, iters, synd);
-        if (rc) return rc;
-        memcpy(m->key, key, sizeof key);
-        m->key_n = nframes;
-    }
-    hipError_t e = hipGraphLaunch(m->gexec, (hipStream_t)stream);
-    return e == hipSuccess ? WL_OK : hipf(e, "graph launch");
+    return run_graph(m, 0, 0, cfg, frame_off, llr, nframes, post, hard, ret, iters, synd, nullptr,
+                     (hipStream_t)stream);
+}
+
+int wl_mixed_set_offsets(wl_mixed* m, int rate, const float* offsets_host, int T)
+{
+    if (!m) return mfail(WL_EINVAL, "null mixed decoder");
+    if (rate < 0 || rate >= kNumRates) return mfail(WL_EINVAL, "invalid rate");
+    if (m->offsets[rate]) (void)hipFree(m->offsets[rate]);
+    m->offsets[rate] = nullptr;
+    m->T[rate] = 0;
+    drop_graph(m);
+    if (!offsets_host) return WL_OK;
+    if (T < 1 || T > 255) return mfail(WL_EINVAL, "T out of range 1..255");
+    const LayerTable lt = make_layer_table(rate);
+    const size_t np = (size_t)T * lt.nl * lt.c;
+    hipError_t e = hipMalloc((void**)&m->offsets[rate], np * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(m->offsets[rate], offsets_host, np * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hipf(e, "offsets upload");
+    m->T[rate] = T;
+    return WL_OK;
+}
+
+int wl_mixed_decode_learned_fallback_dev(wl_mixed* m, const uint8_t* cfg, const uint64_t* frame_off,
+                                         const float* llr, size_t nframes, int fallback_iters, float* post,
+                                         uint8_t* hard, uint8_t* ret, uint8_t* iters, uint16_t* synd,
+                                         uint8_t* fallback, void* stream)
+{
+    if (!m) return mfail(WL_EINVAL, "null mixed decoder");
+    if (nframes == 0) return WL_OK;
+    if (!cfg || !frame_off || !llr) return mfail(WL_EINVAL, "cfg, frame_off and llr are required");
+    if (nframes > 0xffffffffull) return mfail(WL_EINVAL, "nframes too large");
+    if (fallback_iters < 0 || fallback_iters > 255) return mfail(WL_EINVAL, "fallback_iters out of range");
+    return run_graph(m, 1, fallback_iters, cfg, frame_off, llr, nframes, post, hard, ret, iters, synd, fallback,
+                     (hipStream_t)stream);
 }
 
 int wl_mixed_invalid_count(wl_mixed* m, uint64_t* out)

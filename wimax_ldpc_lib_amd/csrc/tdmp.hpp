@@ -43,6 +43,8 @@ struct DecodeArgs {
                                           // (hard: frame_off[g] / 8 bytes); default g * n
     unsigned long long* tile_counter;     // nullable: persistent mode, tiles taken from this counter
     int persistent_grid;                  // persistent mode: workgroups launched (<= tiles)
+    int fallback_iters;                   // MODE_LEARNED_FB: normalised min-sum iterations of re-queued frames
+    uint8_t* fallback;                    // MODE_LEARNED_FB: [nframes] 1 = re-decoded (nullable)
     unsigned long long nframes;
     int frames_per_wg;         // F: frames decoded together by one workgroup
     int max_iter;
@@ -54,10 +56,16 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kLdsPad = 512;   // per-frame counters; also keeps wrapped row bases >= 0
 constexpr int kTileSlot = 504; // persistent mode: the tile index broadcast, inside the pad
+constexpr int kQueueOff = 128;      // MODE_LEARNED_FB: re-queue of failed frame ids (2F uint32)
+constexpr int kFailOff = 304;       // MODE_LEARNED_FB: per-slot return values (F bytes)
+constexpr int kQueueCountOff = 496; // MODE_LEARNED_FB: queue length
 constexpr int kMaxThreads = 512;
 constexpr int kMaxLdsBytes = 65536;
 
-enum { MODE_NMS = 0, MODE_LEARNED = 1 };
+// MODE_LEARNED_FB: learned decoding, then frames whose syndrome fails are
+// re-queued (per workgroup, in LDS) and decoded again from their input LLRs
+// with normalised min-sum in the same persistent kernel.
+enum { MODE_NMS = 0, MODE_LEARNED = 1, MODE_LEARNED_FB = 2 };
 
 // Code-shape variants (bit mask), all bit-identical in results:
 //   VAR_NOHOIST  recompute each slot's wrapped LDS address inside the layer
@@ -77,9 +85,11 @@ constexpr int max_frames_per_wg()
 
 // One tile: frames [frame0, frame0 + Feff) of the (possibly indirect) frame
 // list, decoded by one workgroup.  All threads of the workgroup call it.
+// ids (nullable): explicit frame ids of the tile's slots (overrides the frame
+// list); fail_out (nullable): per-slot return value, for re-queueing.
 template <int RATE, int Z, int MODE, int VAR>
 __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* smem, unsigned long long frame0,
-                                          int Feff)
+                                          int Feff, const uint32_t* ids = nullptr, uint8_t* fail_out = nullptr)
 {
     using CD = Code<RATE, Z>;
     constexpr int NL = CD::NL;
@@ -97,13 +107,14 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     const unsigned long long slot0 = a.bucket ? (unsigned long long)a.bucket[0] : 0ull;
     // global frame of slot s, and its element offset
     auto gframe = [&](int s) -> unsigned long long {
+        if (ids) return (unsigned long long)ids[s];
         return a.frame_idx ? (unsigned long long)a.frame_idx[slot0 + frame0 + s] : frame0 + s;
     };
     auto goff = [&](int s) -> unsigned long long {
         const unsigned long long g = gframe(s);
         return a.frame_off ? a.frame_off[g] : g * N;
     };
-    const bool contiguous = !a.frame_idx && !a.frame_off;
+    const bool contiguous = !a.frame_idx && !a.frame_off && !ids;
 
     if (tid < F) cnt[tid] = 0;
     if (contiguous) {
@@ -289,6 +300,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             if (a.early_stop) rv = (converged || a.max_iter == 0) ? 0 : 1;  // :124,141,261
             else rv = w != 0;
             a.ret[g] = rv;
+            if (fail_out) fail_out[f] = rv;
         }
     }
 
@@ -335,15 +347,63 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
     // persistent: take tiles until the list is exhausted (every workgroup
     // reaches the exit: the counter only grows)
     unsigned long long* slot = reinterpret_cast<unsigned long long*>(smem + kTileSlot);
-    for (;;) {
-        if (threadIdx.x == 0) *slot = atomicAdd(a.tile_counter, 1ull);
-        __syncthreads();
-        const unsigned long long frame0 = *slot * (unsigned long long)F;
-        __syncthreads();
-        if (frame0 >= nfr) break;
-        const unsigned long long left = nfr - frame0;
-        tdmp_tile<RATE, Z, MODE, VAR>(a, smem, frame0, left < (unsigned long long)F ? (int)left : F);
-        __syncthreads();
+    if constexpr (MODE != MODE_LEARNED_FB) {
+        for (;;) {
+            if (threadIdx.x == 0) *slot = atomicAdd(a.tile_counter, 1ull);
+            __syncthreads();
+            const unsigned long long frame0 = *slot * (unsigned long long)F;
+            __syncthreads();
+            if (frame0 >= nfr) break;
+            const unsigned long long left = nfr - frame0;
+            tdmp_tile<RATE, Z, MODE, VAR>(a, smem, frame0, left < (unsigned long long)F ? (int)left : F);
+            __syncthreads();
+        }
+    } else {
+        // learned tiles; failures go to this workgroup's queue (LDS), decoded
+        // F at a time with normalised min-sum, the rest after the last tile
+        uint32_t* queue = reinterpret_cast<uint32_t*>(smem + kQueueOff);   // [2F]
+        uint8_t* fails = smem + kFailOff;                                   // [F]
+        uint32_t* qn = reinterpret_cast<uint32_t*>(smem + kQueueCountOff);
+        if (threadIdx.x == 0) *qn = 0;
+        DecodeArgs b = a;   // the fallback pass
+        b.max_iter = a.fallback_iters;
+        b.early_stop = 1;
+        for (;;) {
+            if (threadIdx.x == 0) *slot = atomicAdd(a.tile_counter, 1ull);
+            __syncthreads();
+            const unsigned long long frame0 = *slot * (unsigned long long)F;
+            const bool last = frame0 >= nfr;
+            __syncthreads();
+            if (!last) {
+                const unsigned long long left = nfr - frame0;
+                const int fe = left < (unsigned long long)F ? (int)left : F;
+                tdmp_tile<RATE, Z, MODE_LEARNED, 0>(a, smem, frame0, fe, nullptr, fails);
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    const unsigned long long s0 = a.bucket ? (unsigned long long)a.bucket[0] : 0ull;
+                    uint32_t n = *qn;
+                    for (int s = 0; s < fe; s++) {
+                        const uint32_t g = a.frame_idx ? a.frame_idx[s0 + frame0 + s] : (uint32_t)(frame0 + s);
+                        if (a.fallback) a.fallback[g] = fails[s];
+                        if (fails[s]) queue[n++] = g;
+                    }
+                    *qn = n;
+                }
+                __syncthreads();
+            }
+            const int q = (int)*qn;
+            if (q >= F || (last && q > 0)) {
+                const int take = q < F ? q : F;
+                tdmp_tile<RATE, Z, MODE_NMS, VAR>(b, smem, 0, take, queue, nullptr);
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    for (int i = take; i < q; i++) queue[i - take] = queue[i];
+                    *qn = (uint32_t)(q - take);
+                }
+                __syncthreads();
+            }
+            if (last && *qn == 0) break;   // *qn is read by all after the barrier above
+        }
     }
 }
 

@@ -50,6 +50,21 @@ class MixedDecoder:
         check(lib().wl_mixed_decode_dev(self._h, _ptr(cfg), _ptr(off), _ptr(llr), cfg.numel(), _ptr(post),
                                         _ptr(hard), _ptr(ret), _ptr(iters), _ptr(synd), _stream(stream)))
 
+    def set_offsets(self, rate, offsets):
+        import numpy as np
+        if offsets is None:
+            check(lib().wl_mixed_set_offsets(self._h, int(rate), None, 0))
+            return
+        o = np.ascontiguousarray(offsets, dtype=np.float32)
+        check(lib().wl_mixed_set_offsets(self._h, int(rate), _ptr(o), o.shape[0]))
+
+    def decode_learned_fallback_dev(self, cfg, off, llr, fallback_iters, post=None, hard=None, ret=None,
+                                    iters=None, synd=None, fallback=None, stream=None):
+        check(lib().wl_mixed_decode_learned_fallback_dev(self._h, _ptr(cfg), _ptr(off), _ptr(llr), cfg.numel(),
+                                                         int(fallback_iters), _ptr(post), _ptr(hard), _ptr(ret),
+                                                         _ptr(iters), _ptr(synd), _ptr(fallback),
+                                                         _stream(stream)))
+
     def invalid_count(self):
         v = C.c_uint64()
         check(lib().wl_mixed_invalid_count(self._h, C.byref(v)))
