@@ -1,0 +1,225 @@
+"""Measurements of the decoder workloads other than the headline bench.
+
+  cfg1   rate 1/2, z = 24, 1024 frames at Eb/N0 2.5 dB, max 10 iterations with
+         the reference's early stop: GPU decode vs the CPU oracle on the same
+         frames (bit-exact check, timing)
+  cfg3   262144 frames, rate uniform over {1/2, 2/3A, 2/3B, 3/4A, 5/6}, z
+         uniform over the 19 values, Eb/N0 uniform in [1.0, 4.5] dB, max 10
+         iterations with early stop, decoded as one mixed batch (one captured
+         graph of per-code persistent kernels); compared with decoding each
+         code's frames by its own launch, bit for bit
+  cfg5   524288 frames, the five encoder rates at z = 96, Eb/N0 uniform in
+         [1.5, 5.0] dB; learned offsets per rate (trained here if missing),
+         learned 5 iterations with in-kernel re-queue of syndrome failures to
+         normalised min-sum (10 iterations), against plain normalised
+         min-sum (10 iterations) on the same frames: info Gbit/s, fallback
+         fraction, BER / FER per rate
+
+    python -m wimax_ldpc_lib_amd.workloads cfg1|cfg3|cfg5 [--out file.json]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(PKG))
+RATES5 = [0, 1, 2, 3, 5]
+K_OF = {0: 12, 1: 16, 2: 16, 3: 18, 5: 20}   # info blocks per rate
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def timed(torch, fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def cfg1(a, torch, W, np):
+    from oracle import oracle as O
+    code = O.Code(0, 24)
+    enc = W.ldpc_encoder(0, 24)
+    F = 1024
+    llr = torch.empty((F, code.n), dtype=torch.float32, device="cuda")
+    enc.channel_dev(a.seed, 0, F, 2.5, llr=llr)
+    dec = W.ldpc_decoder(0, 24, 10, early_stop=True)
+    post = torch.empty_like(llr)
+    hard = torch.empty((F, code.n // 8), dtype=torch.uint8, device="cuda")
+    ret = torch.empty(F, dtype=torch.uint8, device="cuda")
+    iters = torch.empty(F, dtype=torch.uint8, device="cuda")
+    ms = timed(torch, lambda: dec.decode_dev(llr, post, hard, ret, iters), 20)
+    o = code.decode(llr.cpu().numpy(), 10, True, nthreads=O.omp_max_threads())
+    same = (np.array_equal(post.cpu().numpy().view(np.uint32), o["post"].view(np.uint32)) and
+            np.array_equal(hard.cpu().numpy(), np.packbits(o["hd"], axis=1, bitorder="little")) and
+            np.array_equal(iters.cpu().numpy(), o["iters"]) and np.array_equal(ret.cpu().numpy(), o["ret"]))
+    out = {"frames": F, "gpu_ms": ms, "gpu_info_gbps": F * code.k / (ms * 1e-3) / 1e9,
+           "cpu_seconds": o["seconds"], "cpu_threads": O.omp_max_threads(),
+           "cpu_info_gbps": F * code.k / o["seconds"] / 1e9, "bit_exact": bool(same),
+           "iters_histogram": np.bincount(o["iters"], minlength=11).tolist(), "failures": int(o["ret"].sum())}
+    log(json.dumps(out))
+    return out
+
+
+def mixed_inputs(torch, md, np, F, codes, seed, lo, hi):
+    rng = np.random.default_rng(seed)
+    cfg_np = rng.choice(np.array(codes, dtype=np.uint8), F)
+    cfg = torch.from_numpy(cfg_np).cuda()
+    off = torch.empty(F, dtype=torch.int64, device="cuda")
+    md.offsets_dev(cfg, off)
+    n_of = 24 * (24 + 4 * (cfg_np.astype(np.int64) % 19))
+    total = int(n_of.sum())
+    llr = torch.empty(total, dtype=torch.float32, device="cuda")
+    bits = torch.empty(total, dtype=torch.uint8, device="cuda")
+    md.channel_dev(seed, cfg, off, lo, hi, llr, bits)
+    torch.cuda.synchronize()
+    return cfg_np, cfg, off, llr, bits, n_of
+
+
+def cfg3(a, torch, W, np):
+    from wimax_ldpc_lib_amd.mixed import MixedDecoder, code_id
+    F = a.frames or 262144
+    codes = [code_id(r, z) for r in RATES5 for z in range(24, 97, 4)]
+    md = MixedDecoder(max_iter=10, early_stop=True, codes=codes)
+    cfg_np, cfg, off, llr, bits, n_of = mixed_inputs(torch, md, np, F, codes, a.seed, 1.0, 4.5)
+    total = llr.numel()
+    k_of = np.array([K_OF[int(c) // 19] * (24 + 4 * (int(c) % 19)) for c in cfg_np], dtype=np.int64)
+    info_bits = int(k_of.sum())
+    post = torch.empty_like(llr)
+    hard = torch.empty(total // 8, dtype=torch.uint8, device="cuda")
+    ret = torch.empty(F, dtype=torch.uint8, device="cuda")
+    iters = torch.empty(F, dtype=torch.uint8, device="cuda")
+    ms = timed(torch, lambda: md.decode_dev(cfg, off, llr, None, hard, ret, iters), a.reps)
+    log("mixed graph: %.3f ms, %.2f info Gbit/s" % (ms, info_bits / (ms * 1e-3) / 1e9))
+    hard_m, ret_m, it_m = hard.cpu().numpy(), ret.cpu().numpy(), iters.cpu().numpy()
+    # per-code launches over gathered frames, and the comparison
+    llr_np = llr.cpu().numpy()
+    offs = off.cpu().numpy()
+    per = []
+    same = True
+    tot_ms = 0.0
+    for c in codes:
+        sel = np.flatnonzero(cfg_np == c)
+        if not len(sel):
+            continue
+        rate, z = c // 19, 24 + 4 * (c % 19)
+        n = 24 * z
+        frames = torch.from_numpy(np.stack([llr_np[offs[g]:offs[g] + n] for g in sel])).cuda()
+        dec = W.ldpc_decoder(rate, z, 10, early_stop=True)
+        h1 = torch.empty((len(sel), n // 8), dtype=torch.uint8, device="cuda")
+        r1 = torch.empty(len(sel), dtype=torch.uint8, device="cuda")
+        i1 = torch.empty(len(sel), dtype=torch.uint8, device="cuda")
+        tot_ms += timed(torch, lambda: dec.decode_dev(frames, None, h1, r1, i1), 3)
+        h1 = h1.cpu().numpy()
+        for q, g in enumerate(sel):
+            if not np.array_equal(hard_m[offs[g] // 8:(offs[g] + n) // 8], h1[q]):
+                same = False
+        same = same and np.array_equal(ret_m[sel], r1.cpu().numpy()) and np.array_equal(it_m[sel], i1.cpu().numpy())
+        per.append(len(sel))
+    out = {"frames": F, "codes": len(per), "info_bits": info_bits, "mixed_ms": ms,
+           "mixed_info_gbps": info_bits / (ms * 1e-3) / 1e9, "per_code_launches_ms_sum": tot_ms,
+           "per_code_info_gbps": info_bits / (tot_ms * 1e-3) / 1e9, "bit_exact_vs_per_code": bool(same),
+           "avg_iters": float(it_m.mean()), "syndrome_fail_rate": float(ret_m.mean()),
+           "frames_per_code_min_max": [int(min(per)), int(max(per))]}
+    log(json.dumps(out))
+    return out
+
+
+def offsets_for(rate, a, np, log):
+    path = os.path.join(PKG, "trained", "offsets_r%d_z96_T5.json" % rate)
+    if os.path.exists(path):
+        return np.array(json.load(open(path))["offsets"], dtype=np.float32), path
+    from wimax_ldpc_lib_amd.learned import Trainer
+    tr = Trainer(rate, 96, T=5, batch=4096, seed=a.seed + rate, lr=1e-3, init_offset=0.15)
+    t0 = time.time()
+    for i in range(a.train_steps // 1000):
+        loss = tr.step(1000, 1.0, 4.0)
+        log("rate %d step %d loss %.5f (%.0f s)" % (rate, (i + 1) * 1000, loss, time.time() - t0))
+    o = tr.offsets()
+    if a.save_offsets:
+        json.dump({"rate": rate, "z": 96, "T": 5, "offsets": o.tolist(),
+                   "training": {"steps": a.train_steps, "batch": 4096, "lr": 1e-3, "init_offset": 0.15,
+                                "ebn0_range_db": [1.0, 4.0], "seconds": time.time() - t0}}, open(path, "w"))
+    return o, path
+
+
+def cfg5(a, torch, W, np):
+    from wimax_ldpc_lib_amd.mixed import MixedDecoder, code_id
+    F = a.frames or 524288
+    codes = [code_id(r, 96) for r in RATES5]
+    md_l = MixedDecoder(max_iter=5, early_stop=True, codes=codes)
+    md_p = MixedDecoder(max_iter=10, early_stop=True, codes=codes)
+    srcs = {}
+    for r in RATES5:
+        o, src = offsets_for(r, a, np, log)
+        md_l.set_offsets(r, o)
+        srcs[r] = os.path.basename(src)
+    cfg_np, cfg, off, llr, bits, n_of = mixed_inputs(torch, md_l, np, F, codes, a.seed + 5, 1.5, 5.0)
+    k_of = np.array([K_OF[int(c) // 19] * 96 for c in cfg_np], dtype=np.int64)
+    info_bits = int(k_of.sum())
+    hard = torch.empty(F * 288, dtype=torch.uint8, device="cuda")
+    ret = torch.empty(F, dtype=torch.uint8, device="cuda")
+    iters = torch.empty(F, dtype=torch.uint8, device="cuda")
+    fb = torch.empty(F, dtype=torch.uint8, device="cuda")
+    ms_l = timed(torch, lambda: md_l.decode_learned_fallback_dev(cfg, off, llr, 10, None, hard, ret, iters, None, fb),
+                 a.reps)
+    res = {}
+    bits2 = bits.view(F, 2304)
+    weights = (1 << torch.arange(8, dtype=torch.int32, device="cuda")).view(1, 1, 8)
+    ref = (bits2.view(F, 288, 8).int() * weights).sum(dim=2).to(torch.uint8)
+    popc = torch.tensor([bin(i).count("1") for i in range(256)], dtype=torch.int32, device="cuda")
+
+    def per_rate(hard_t):
+        x = hard_t.view(F, 288) ^ ref
+        out = {}
+        for r in RATES5:
+            sel = torch.from_numpy(np.flatnonzero(cfg_np == code_id(r, 96))).cuda()
+            kb = K_OF[r] * 96 // 8
+            e = popc[x[sel, :kb].long()].sum(dim=1)
+            out[str(r)] = {"frames": int(sel.numel()), "ber": float(e.sum()) / (sel.numel() * K_OF[r] * 96),
+                           "fer": float((e > 0).float().mean())}
+        return out
+    res["learned_fallback"] = {"ms": ms_l, "info_gbps": info_bits / (ms_l * 1e-3) / 1e9,
+                               "fallback_fraction": float(fb.float().mean()),
+                               "fallback_fraction_per_rate": {str(r): float(fb[torch.from_numpy(
+                                   np.flatnonzero(cfg_np == code_id(r, 96))).cuda()].float().mean()) for r in RATES5},
+                               "avg_iters": float(iters.float().mean()), "per_rate": per_rate(hard)}
+    ms_p = timed(torch, lambda: md_p.decode_dev(cfg, off, llr, None, hard, ret, iters), a.reps)
+    res["nms10"] = {"ms": ms_p, "info_gbps": info_bits / (ms_p * 1e-3) / 1e9, "avg_iters": float(iters.float().mean()),
+                    "per_rate": per_rate(hard)}
+    out = {"frames": F, "info_bits": info_bits, "offsets": srcs, **res,
+           "speedup_learned_fallback_vs_nms10": ms_p / ms_l}
+    log(json.dumps(out))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("which", choices=["cfg1", "cfg3", "cfg5"])
+    ap.add_argument("--frames", type=int, default=0)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--seed", type=int, default=31337)
+    ap.add_argument("--train-steps", type=int, default=20000)
+    ap.add_argument("--save-offsets", action="store_true")
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    import numpy as np
+    import torch
+
+    import wimax_ldpc_lib_amd as W
+    out = {"cfg1": cfg1, "cfg3": cfg3, "cfg5": cfg5}[a.which](a, torch, W, np)
+    if a.out:
+        json.dump(out, open(a.out, "w"), indent=1)
+
+
+if __name__ == "__main__":
+    main()
