@@ -34,7 +34,7 @@ constexpr DecodeLaunch fb_launcher()
 template <int RATE, int Z>
 constexpr DecodeOccupancy fb_occupancy()
 {
-    if constexpr (Z == 24 || Z == 96) return &occupancy_tdmp<RATE, Z, MODE_LEARNED_FB, VAR_NOHOIST>;
+    if constexpr (Z == 24 || Z == 96) return &occupancy_tdmp<RATE, Z, MODE_LEARNED_FB, VAR_NOHOIST, true>;
     else return nullptr;
 }
 
@@ -69,9 +69,9 @@ constexpr auto make_decode_table(std::integer_sequence<int, ZI...>)
                              &launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_NOHOIST>},
                             &launch_tdmp<RATE, z_of_index(ZI), MODE_LEARNED>,
                             max_frames_per_wg<RATE, z_of_index(ZI)>(),
-                            {&occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0>,
-                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_NOHOIST>},
-                            &occupancy_tdmp<RATE, z_of_index(ZI), MODE_LEARNED>,
+                            {&occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0, true>,
+                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_NOHOIST, true>},
+                            &occupancy_tdmp<RATE, z_of_index(ZI), MODE_LEARNED, 0, true>,
                             fb_launcher<RATE, z_of_index(ZI)>(),
                             fb_occupancy<RATE, z_of_index(ZI)>()}...}};
 }

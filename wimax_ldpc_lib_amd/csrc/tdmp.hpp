@@ -87,7 +87,10 @@ constexpr int max_frames_per_wg()
 // list, decoded by one workgroup.  All threads of the workgroup call it.
 // ids (nullable): explicit frame ids of the tile's slots (overrides the frame
 // list); fail_out (nullable): per-slot return value, for re-queueing.
-template <int RATE, int Z, int MODE, int VAR>
+// EXT = false compiles the plain form only (contiguous frames, none of the
+// indirection below), which keeps scalar-register pressure of the headline
+// kernel at its minimum.
+template <int RATE, int Z, int MODE, int VAR, bool EXT>
 __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* smem, unsigned long long frame0,
                                           int Feff, const uint32_t* ids = nullptr, uint8_t* fail_out = nullptr)
 {
@@ -104,17 +107,23 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     const int f = tid / Z;
     const int r = tid - f * Z;
     const bool active = f < Feff;
-    const unsigned long long slot0 = a.bucket ? (unsigned long long)a.bucket[0] : 0ull;
     // global frame of slot s, and its element offset
     auto gframe = [&](int s) -> unsigned long long {
-        if (ids) return (unsigned long long)ids[s];
-        return a.frame_idx ? (unsigned long long)a.frame_idx[slot0 + frame0 + s] : frame0 + s;
+        if constexpr (EXT) {
+            if (ids) return (unsigned long long)ids[s];
+            const unsigned long long slot0 = a.bucket ? (unsigned long long)a.bucket[0] : 0ull;
+            return a.frame_idx ? (unsigned long long)a.frame_idx[slot0 + frame0 + s] : frame0 + s;
+        } else {
+            return frame0 + s;
+        }
     };
     auto goff = [&](int s) -> unsigned long long {
         const unsigned long long g = gframe(s);
-        return a.frame_off ? a.frame_off[g] : g * N;
+        if constexpr (EXT) return a.frame_off ? a.frame_off[g] : g * N;
+        else return g * N;
     };
-    const bool contiguous = !a.frame_idx && !a.frame_off && !ids;
+    bool contiguous = true;
+    if constexpr (EXT) contiguous = !a.frame_idx && !a.frame_off && !ids;
 
     if (tid < F) cnt[tid] = 0;
     if (contiguous) {
@@ -300,7 +309,9 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             if (a.early_stop) rv = (converged || a.max_iter == 0) ? 0 : 1;  // :124,141,261
             else rv = w != 0;
             a.ret[g] = rv;
-            if (fail_out) fail_out[f] = rv;
+            if constexpr (EXT) {
+                if (fail_out) fail_out[f] = rv;
+            }
         }
     }
 
@@ -327,35 +338,47 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     }
 }
 
-template <int RATE, int Z, int MODE, int VAR = 0>
+template <int RATE, int Z, int MODE, int VAR = 0, bool EXT = false>
 __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int F = a.frames_per_wg;
+    if constexpr (!EXT) {
+        // plain launch: tile blockIdx.x of a contiguous frame range
+        const unsigned long long frame0 = (unsigned long long)blockIdx.x * (unsigned long long)F;
+        if (frame0 >= a.nframes) return;   // whole workgroup, before any barrier
+        const unsigned long long left = a.nframes - frame0;
+        tdmp_tile<RATE, Z, MODE, VAR, false>(a, smem, frame0, left < (unsigned long long)F ? (int)left : F);
+        return;
+    }
     unsigned long long nfr = a.nframes;
     if (a.bucket) {
         const unsigned long long c = a.bucket[1];
         nfr = c < nfr ? c : nfr;
     }
-    if (!a.tile_counter) {
-        const unsigned long long frame0 = (unsigned long long)blockIdx.x * (unsigned long long)F;
-        if (frame0 >= nfr) return;   // whole workgroup, before any barrier
-        const unsigned long long left = nfr - frame0;
-        tdmp_tile<RATE, Z, MODE, VAR>(a, smem, frame0, left < (unsigned long long)F ? (int)left : F);
-        return;
-    }
-    // persistent: take tiles until the list is exhausted (every workgroup
-    // reaches the exit: the counter only grows)
+    // One call site of the tile body for both launch modes: the plain launch
+    // decodes tile blockIdx.x once; the persistent one takes tiles from the
+    // counter until the list is exhausted (every workgroup reaches the exit:
+    // the counter only grows).
     unsigned long long* slot = reinterpret_cast<unsigned long long*>(smem + kTileSlot);
     if constexpr (MODE != MODE_LEARNED_FB) {
+        bool first = true;
         for (;;) {
-            if (threadIdx.x == 0) *slot = atomicAdd(a.tile_counter, 1ull);
-            __syncthreads();
-            const unsigned long long frame0 = *slot * (unsigned long long)F;
-            __syncthreads();
-            if (frame0 >= nfr) break;
+            unsigned long long tile;
+            if (a.tile_counter) {
+                if (threadIdx.x == 0) *slot = atomicAdd(a.tile_counter, 1ull);
+                __syncthreads();
+                tile = *slot;
+                __syncthreads();
+            } else {
+                if (!first) break;
+                tile = blockIdx.x;
+            }
+            first = false;
+            const unsigned long long frame0 = tile * (unsigned long long)F;
+            if (frame0 >= nfr) break;   // uniform over the workgroup
             const unsigned long long left = nfr - frame0;
-            tdmp_tile<RATE, Z, MODE, VAR>(a, smem, frame0, left < (unsigned long long)F ? (int)left : F);
+            tdmp_tile<RATE, Z, MODE, VAR, true>(a, smem, frame0, left < (unsigned long long)F ? (int)left : F);
             __syncthreads();
         }
     } else {
@@ -377,7 +400,7 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
             if (!last) {
                 const unsigned long long left = nfr - frame0;
                 const int fe = left < (unsigned long long)F ? (int)left : F;
-                tdmp_tile<RATE, Z, MODE_LEARNED, 0>(a, smem, frame0, fe, nullptr, fails);
+                tdmp_tile<RATE, Z, MODE_LEARNED, 0, true>(a, smem, frame0, fe, nullptr, fails);
                 __syncthreads();
                 if (threadIdx.x == 0) {
                     const unsigned long long s0 = a.bucket ? (unsigned long long)a.bucket[0] : 0ull;
@@ -394,7 +417,7 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
             const int q = (int)*qn;
             if (q >= F || (last && q > 0)) {
                 const int take = q < F ? q : F;
-                tdmp_tile<RATE, Z, MODE_NMS, VAR>(b, smem, 0, take, queue, nullptr);
+                tdmp_tile<RATE, Z, MODE_NMS, VAR, true>(b, smem, 0, take, queue, nullptr);
                 __syncthreads();
                 if (threadIdx.x == 0) {
                     for (int i = take; i < q; i++) queue[i - take] = queue[i];
@@ -408,18 +431,20 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
 }
 
 // Workgroups of this kernel resident per CU at frames_per_wg F (0 on error).
-template <int RATE, int Z, int MODE, int VAR = 0>
+template <int RATE, int Z, int MODE, int VAR = 0, bool EXT = false>
 int occupancy_tdmp(int F)
 {
     const int threads = ((F * Z + 63) / 64) * 64;
     const size_t lds = kLdsPad + (size_t)F * Code<RATE, Z>::N * sizeof(float);
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tdmp_kernel<RATE, Z, MODE, VAR>, threads, lds) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tdmp_kernel<RATE, Z, MODE, VAR, EXT>, threads, lds) !=
         hipSuccess)
         return 0;
     return nb;
 }
 
+// EXT kernels serve indirect, ragged and persistent launches; the plain one
+// contiguous batches.  The launcher picks by the arguments.
 template <int RATE, int Z, int MODE, int VAR = 0>
 hipError_t launch_tdmp(const DecodeArgs& a, hipStream_t stream)
 {
@@ -432,7 +457,11 @@ hipError_t launch_tdmp(const DecodeArgs& a, hipStream_t stream)
     if (a.tile_counter && a.persistent_grid > 0 && (unsigned long long)a.persistent_grid < grid)
         grid = (unsigned long long)a.persistent_grid;
     if (grid > 0x7fffffffull) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((tdmp_kernel<RATE, Z, MODE, VAR>), dim3((unsigned)grid), dim3(threads), lds, stream, a);
+    const bool ext = MODE == MODE_LEARNED_FB || a.frame_idx || a.bucket || a.frame_off || a.tile_counter;
+    if (ext)
+        hipLaunchKernelGGL((tdmp_kernel<RATE, Z, MODE, VAR, true>), dim3((unsigned)grid), dim3(threads), lds, stream, a);
+    else if constexpr (MODE != MODE_LEARNED_FB)
+        hipLaunchKernelGGL((tdmp_kernel<RATE, Z, MODE, VAR, false>), dim3((unsigned)grid), dim3(threads), lds, stream, a);
     return hipGetLastError();
 }
 
