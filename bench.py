@@ -93,13 +93,21 @@ def main():
     log("kernel avg %.3f ms (min %.3f), step %.3f ms, %.2f info Gbit/s, info BER %.3g"
         % (kavg, min(kern_ms), ms_step, value, ber))
 
-    traffic = None
+    # HBM bytes and VALU issue of this kernel from the committed rocprofv3
+    # counter runs (FETCH_SIZE x2 and WRITE_SIZE x1 after copy calibration)
+    traffic, valu = None, None
     pmc = os.path.join(ROOT, "profiles", "tdmp_r5z96_f65536_traffic.json")
     if os.path.exists(pmc):
-        try:
-            traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+        prof = json.load(open(pmc))
+        traffic = prof.get("hbm_bytes_per_launch")
+        if prof.get("valu_wave_insts_per_launch"):
+            # issue rate over this run's kernel time: wave instructions per CU per cycle
+            # at the 2.4 GHz peak engine clock (peak 1.0: 4 SIMDs x 1/4 per cycle)
+            ipc = prof["valu_wave_insts_per_launch"] / (prof["num_cu"] * 2.4e9 * kavg * 1e-3)
+            valu = {"wave_insts_per_launch": prof["valu_wave_insts_per_launch"],
+                    "issue_per_cu_cycle_at_2.4GHz": ipc,
+                    "issue_frac_counters": prof.get("valu_issue_frac"),
+                    "note": "the kernel is bound by VALU issue, not HBM"}
 
     cpu = None
     if not args.no_cpu:
@@ -140,7 +148,7 @@ def main():
                    "frames": F, "frames_per_wg": dec.frames_per_wg, "iterations": ITERS},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel_ms": kavg, "bytes_per_frame": bytes_per_frame},
+                     "kernel_ms": kavg, "bytes_per_frame": bytes_per_frame, "valu": valu},
         "cpu_baseline": cpu,
         "info_ber": ber,
     }

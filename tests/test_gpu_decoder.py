@@ -42,7 +42,7 @@ def _compare(g, o, label):
             label, bad.size, bad[:4], g["post"].ravel()[bad[:4]], o["post"].ravel()[bad[:4]])
 
 
-def test_cfg1_rate12_z24_early_stop(gpu, oracle_mod):
+def test_rate12_z24_1024_frames_early_stop(gpu, oracle_mod):
     """Rate 1/2, z=24, 1024 frames at 2.5 dB, max 10 iterations, early stop."""
     import wimax_ldpc_lib_amd as W
     code = oracle_mod.Code(0, 24)
@@ -50,18 +50,18 @@ def test_cfg1_rate12_z24_early_stop(gpu, oracle_mod):
     o = code.decode(llr, max_iter=10, early_stop=True)
     dec = W.ldpc_decoder(0, 24, 10, early_stop=True)
     g = _gpu_decode(gpu, dec, llr)
-    _compare(g, o, "cfg1")
+    _compare(g, o, "rate 1/2 z=24")
     assert o["ret"].sum() >= 1 and len(np.unique(o["iters"])) > 3   # failures and varied stops
 
 
-def test_cfg2_shape_fixed10(gpu, oracle_mod):
+def test_rate56_z96_fixed10(gpu, oracle_mod):
     """Rate 5/6, z=96, exactly 10 iterations (no early stop), 512 frames at 4.0 dB."""
     import wimax_ldpc_lib_amd as W
     code = oracle_mod.Code(5, 96)
     llr, _ = noisy_llr(code, 512, 4.0, seed=12)
     o = code.decode(llr, max_iter=10, early_stop=False)
     dec = W.ldpc_decoder(5, 96, 10, early_stop=False)
-    _compare(_gpu_decode(gpu, dec, llr), o, "cfg2")
+    _compare(_gpu_decode(gpu, dec, llr), o, "rate 5/6 z=96")
 
 
 @pytest.mark.parametrize("rate", range(6))

@@ -1,21 +1,21 @@
 """Measurements of the decoder workloads other than the headline bench.
 
-  cfg1   rate 1/2, z = 24, 1024 frames at Eb/N0 2.5 dB, max 10 iterations with
+  small  rate 1/2, z = 24, 1024 frames at Eb/N0 2.5 dB, max 10 iterations with
          the reference's early stop: GPU decode vs the CPU oracle on the same
          frames (bit-exact check, timing)
-  cfg3   262144 frames, rate uniform over {1/2, 2/3A, 2/3B, 3/4A, 5/6}, z
+  mixed  262144 frames, rate uniform over {1/2, 2/3A, 2/3B, 3/4A, 5/6}, z
          uniform over the 19 values, Eb/N0 uniform in [1.0, 4.5] dB, max 10
          iterations with early stop, decoded as one mixed batch (one captured
          graph of per-code persistent kernels); compared with decoding each
          code's frames by its own launch, bit for bit
-  cfg5   524288 frames, the five encoder rates at z = 96, Eb/N0 uniform in
+  fallback 524288 frames, the five encoder rates at z = 96, Eb/N0 uniform in
          [1.5, 5.0] dB; learned offsets per rate (trained here if missing),
          learned 5 iterations with in-kernel re-queue of syndrome failures to
          normalised min-sum (10 iterations), against plain normalised
          min-sum (10 iterations) on the same frames: info Gbit/s, fallback
          fraction, BER / FER per rate
 
-    python -m wimax_ldpc_lib_amd.workloads cfg1|cfg3|cfg5 [--out file.json]
+    python -m wimax_ldpc_lib_amd.workloads small|mixed|fallback [--out file.json]
 """
 import argparse
 import json
@@ -45,7 +45,7 @@ def timed(torch, fn, reps):
     return s.elapsed_time(e) / reps
 
 
-def cfg1(a, torch, W, np):
+def small(a, torch, W, np):
     from oracle import oracle as O
     code = O.Code(0, 24)
     enc = W.ldpc_encoder(0, 24)
@@ -85,7 +85,7 @@ def mixed_inputs(torch, md, np, F, codes, seed, lo, hi):
     return cfg_np, cfg, off, llr, bits, n_of
 
 
-def cfg3(a, torch, W, np):
+def mixed(a, torch, W, np):
     from wimax_ldpc_lib_amd.mixed import MixedDecoder, code_id
     F = a.frames or 262144
     codes = [code_id(r, z) for r in RATES5 for z in range(24, 97, 4)]
@@ -152,7 +152,7 @@ def offsets_for(rate, a, np, log):
     return o, path
 
 
-def cfg5(a, torch, W, np):
+def fallback(a, torch, W, np):
     from wimax_ldpc_lib_amd.mixed import MixedDecoder, code_id
     F = a.frames or 524288
     codes = [code_id(r, 96) for r in RATES5]
@@ -204,7 +204,7 @@ def cfg5(a, torch, W, np):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["cfg1", "cfg3", "cfg5"])
+    ap.add_argument("which", choices=["small", "mixed", "fallback"])
     ap.add_argument("--frames", type=int, default=0)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--seed", type=int, default=31337)
@@ -216,7 +216,7 @@ def main():
     import torch
 
     import wimax_ldpc_lib_amd as W
-    out = {"cfg1": cfg1, "cfg3": cfg3, "cfg5": cfg5}[a.which](a, torch, W, np)
+    out = {"small": small, "mixed": mixed, "fallback": fallback}[a.which](a, torch, W, np)
     if a.out:
         json.dump(out, open(a.out, "w"), indent=1)
 
