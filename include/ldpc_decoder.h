@@ -40,6 +40,10 @@ namespace wimax_ldpc_lib {
              * iters [nframes] iteration counts, nullable). */
             void decode_batch(float* rx, size_t nframes, int8_t* decoded, uint8_t* ret, uint8_t* iters);
 
+            /* Extension: decode(int8_t* rx, int8_t* decoded) for nframes
+             * frames of 0/1 bytes in one GPU launch (rx is not modified). */
+            void decode_batch(const int8_t* rx, size_t nframes, int8_t* decoded, uint8_t* ret, uint8_t* iters);
+
             /* Extension: false = exactly max_iter iterations, no syndrome checks. */
             void set_early_stop(bool early_stop);
 

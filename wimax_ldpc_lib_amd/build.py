@@ -4,7 +4,9 @@
     python -m wimax_ldpc_lib_amd.build --force  # rebuild everything
 
 Objects go to wimax_ldpc_lib_amd/_build/, the library to
-wimax_ldpc_lib_amd/lib/libwimax_ldpc_amd.so (both in-tree, git-ignored).
+wimax_ldpc_lib_amd/lib/libwimax_ldpc_amd.so, the drop-in command-line tools
+(test_encoder, test_decoder) to wimax_ldpc_lib_amd/bin/ and the C++ check
+program of the tests to tests/cpp/_bin/ (all in-tree, git-ignored).
 """
 import argparse
 import concurrent.futures as cf
@@ -34,12 +36,23 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "i
 HIP_SOURCES = ["decode_r0.hip", "decode_r1.hip", "decode_r2.hip", "decode_r3.hip", "decode_r4.hip",
                "decode_r5.hip", "encoder.hip", "capi.hip", "learned.hip", "trainer.hip", "mixed.hip"]
 CXX_SOURCES = ["ldpc_classes.cpp"]
+BINDIR = os.path.join(PKG, "bin")
+# host programs linked against the library: (source, output, rpath relative to the output)
+PROGRAMS = [
+    (os.path.join(PKG, "cli", "test_encoder.cpp"), os.path.join(BINDIR, "test_encoder"), "$ORIGIN/../lib"),
+    (os.path.join(PKG, "cli", "test_decoder.cpp"), os.path.join(BINDIR, "test_decoder"), "$ORIGIN/../lib"),
+    (os.path.join(ROOT, "tests", "cpp", "drop_in_check.cpp"), os.path.join(ROOT, "tests", "cpp", "_bin", "drop_in_check"),
+     "$ORIGIN/../../../wimax_ldpc_lib_amd/lib"),
+]
 
 
-def _headers():
+def _headers(hip=False):
+    """Headers a source depends on: the HIP sources see csrc/ and the C ABI
+    header only, host C++ sources every public header."""
     hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h", ".inc"))]
     inc = os.path.join(ROOT, "include")
-    hs += [os.path.join(inc, f) for f in os.listdir(inc) if f.endswith(".h")]
+    hs += [os.path.join(inc, f) for f in os.listdir(inc)
+           if f.endswith(".h") and (not hip or f == "wimax_ldpc_amd.h")]
     return hs
 
 
@@ -61,13 +74,14 @@ def build(force=False, jobs=None, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     headers = _headers()
+    hip_headers = _headers(hip=True)
     cmds = []
     objs = []
     for s in HIP_SOURCES:
         src = os.path.join(CSRC, s)
         obj = os.path.join(BUILD, s + ".o")
         objs.append(obj)
-        if force or _stale(obj, src, headers):
+        if force or _stale(obj, src, hip_headers):
             cmds.append([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
     for s in CXX_SOURCES:
         src = os.path.join(CSRC, s)
@@ -86,6 +100,13 @@ def build(force=False, jobs=None, verbose=False):
         tmp = LIB + ".tmp"
         _compile([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
         os.replace(tmp, LIB)
+    for src, out, rpath in PROGRAMS:
+        if not os.path.exists(src):
+            continue
+        if force or _stale(out, src, headers + [LIB]):
+            os.makedirs(os.path.dirname(out), exist_ok=True)
+            _compile(["g++"] + CXX_FLAGS + ["-fopenmp", src, "-o", out, "-L" + LIBDIR, "-lwimax_ldpc_amd",
+                                            "-Wl,-rpath," + rpath, "-Wl,-rpath-link,/opt/rocm/lib"])
     return LIB
 
 

@@ -109,6 +109,17 @@ void ldpc_decoder::decode_batch(float* rx, size_t nframes, int8_t* decoded, uint
     check(wl_decode_f32_host(m_handle, rx, nframes, decoded, ret, iters), "decode_batch");
 }
 
+void ldpc_decoder::decode_batch(const int8_t* rx, size_t nframes, int8_t* decoded, uint8_t* ret,
+                                uint8_t* iters)
+{
+    std::vector<uint8_t> tmp;
+    if (!ret) {
+        tmp.resize(nframes);
+        ret = tmp.data();
+    }
+    check(wl_decode_i8_host(m_handle, rx, nframes, decoded, ret, iters), "decode_batch");
+}
+
 void ldpc_decoder::set_early_stop(bool early_stop)
 {
     // a new handle with the other flag; frames_per_wg carries over
