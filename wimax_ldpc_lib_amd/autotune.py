@@ -72,7 +72,12 @@ def main():
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--mbit", type=float, default=150.0)
     ap.add_argument("--out-dir", default=PKG)
+    ap.add_argument("--from-json", default=None,
+                    help="skip measuring: write the table from a tuned.json of an earlier run")
     a = ap.parse_args()
+    if a.from_json:
+        write_outputs(json.load(open(a.from_json)), a.out_dir, a.iters)
+        return
     import torch
 
     import wimax_ldpc_lib_amd as W
@@ -87,13 +92,18 @@ def main():
             b = r["best"]
             print("rate %d z %2d: variant %d F %2d  %.3f ms  %.2f info Gbit/s  (%.0f s)"
                   % (rate, z, b["variant"], b["f"], b["ms"], b["info_gbps"], time.time() - t0), flush=True)
-    with open(os.path.join(a.out_dir, "tuned.json"), "w") as fh:
+    write_outputs(out, a.out_dir, a.iters)
+
+
+def write_outputs(out, out_dir, iters):
+    os.makedirs(os.path.join(out_dir, "csrc"), exist_ok=True)
+    with open(os.path.join(out_dir, "tuned.json"), "w") as fh:
         json.dump(out, fh, indent=0)
     lines = ["// (rate, z, frames_per_wg, variant): fastest launch shape measured on MI355X by",
-             "// wimax_ldpc_lib_amd/autotune.py (%d-iteration normalised min-sum); generated file." % a.iters]
+             "// wimax_ldpc_lib_amd/autotune.py (%d-iteration normalised min-sum); generated file." % iters]
     for r in out:
         lines.append("{%d, %d, %d, %d}," % (r["rate"], r["z"], r["best"]["f"], r["best"]["variant"]))
-    with open(os.path.join(a.out_dir, "csrc", "tuned_table.inc"), "w") as fh:
+    with open(os.path.join(out_dir, "csrc", "tuned_table.inc"), "w") as fh:
         fh.write("\n".join(lines) + "\n")
 
 

@@ -11,6 +11,7 @@ program of the tests to tests/cpp/_bin/ (all in-tree, git-ignored).
 import argparse
 import concurrent.futures as cf
 import os
+import re
 import subprocess
 import sys
 
@@ -46,21 +47,33 @@ PROGRAMS = [
 ]
 
 
-def _headers(hip=False):
-    """Headers a source depends on: the HIP sources see csrc/ and the C ABI
-    header only, host C++ sources every public header."""
-    hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h", ".inc"))]
-    inc = os.path.join(ROOT, "include")
-    hs += [os.path.join(inc, f) for f in os.listdir(inc)
-           if f.endswith(".h") and (not hip or f == "wimax_ldpc_amd.h")]
-    return hs
+_INC = re.compile(r'^\s*#\s*include\s*"([^"]+)"', re.M)
 
 
-def _stale(obj, src, headers):
+def _deps(src, seen=None):
+    """Files a source includes with #include "...", transitively (resolved
+    against its own directory and include/)."""
+    seen = set() if seen is None else seen
+    try:
+        text = open(src).read()
+    except OSError:
+        return seen
+    for name in _INC.findall(text):
+        for d in (os.path.dirname(src), os.path.join(ROOT, "include")):
+            p = os.path.normpath(os.path.join(d, name))
+            if os.path.exists(p):
+                if p not in seen:
+                    seen.add(p)
+                    _deps(p, seen)
+                break
+    return seen
+
+
+def _stale(obj, src, extra=()):
     if not os.path.exists(obj):
         return True
     t = os.path.getmtime(obj)
-    return any(os.path.getmtime(p) > t for p in [src] + headers)
+    return any(os.path.getmtime(p) > t for p in [src] + sorted(_deps(src)) + list(extra))
 
 
 def _compile(cmd):
@@ -73,15 +86,13 @@ def _compile(cmd):
 def build(force=False, jobs=None, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
-    headers = _headers()
-    hip_headers = _headers(hip=True)
     cmds = []
     objs = []
     for s in HIP_SOURCES:
         src = os.path.join(CSRC, s)
         obj = os.path.join(BUILD, s + ".o")
         objs.append(obj)
-        if force or _stale(obj, src, hip_headers):
+        if force or _stale(obj, src):
             cmds.append([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
     for s in CXX_SOURCES:
         src = os.path.join(CSRC, s)
@@ -89,7 +100,7 @@ def build(force=False, jobs=None, verbose=False):
             continue
         obj = os.path.join(BUILD, s + ".o")
         objs.append(obj)
-        if force or _stale(obj, src, headers):
+        if force or _stale(obj, src):
             cmds.append(["g++"] + CXX_FLAGS + ["-c", src, "-o", obj])
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(jobs) as ex:
@@ -103,7 +114,7 @@ def build(force=False, jobs=None, verbose=False):
     for src, out, rpath in PROGRAMS:
         if not os.path.exists(src):
             continue
-        if force or _stale(out, src, headers + [LIB]):
+        if force or _stale(out, src, [LIB]):
             os.makedirs(os.path.dirname(out), exist_ok=True)
             _compile(["g++"] + CXX_FLAGS + ["-fopenmp", src, "-o", out, "-L" + LIBDIR, "-lwimax_ldpc_amd",
                                             "-Wl,-rpath," + rpath, "-Wl,-rpath-link,/opt/rocm/lib"])

@@ -27,14 +27,14 @@ struct DecodeEntry {
 template <int RATE, int Z>
 constexpr DecodeLaunch fb_launcher()
 {
-    if constexpr (Z == 24 || Z == 96) return &launch_tdmp<RATE, Z, MODE_LEARNED_FB, VAR_NOHOIST>;
+    if constexpr (Z == 24 || Z == 96) return &launch_tdmp<RATE, Z, MODE_LEARNED_FB, VAR_LEAN>;
     else return nullptr;
 }
 
 template <int RATE, int Z>
 constexpr DecodeOccupancy fb_occupancy()
 {
-    if constexpr (Z == 24 || Z == 96) return &occupancy_tdmp<RATE, Z, MODE_LEARNED_FB, VAR_NOHOIST, true>;
+    if constexpr (Z == 24 || Z == 96) return &occupancy_tdmp<RATE, Z, MODE_LEARNED_FB, VAR_LEAN, true>;
     else return nullptr;
 }
 
@@ -66,11 +66,11 @@ constexpr auto make_decode_table(std::integer_sequence<int, ZI...>)
 {
     struct Arr { DecodeEntry e[sizeof...(ZI)]; };
     return Arr{{DecodeEntry{{&launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0>,
-                             &launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_NOHOIST>},
+                             &launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN>},
                             &launch_tdmp<RATE, z_of_index(ZI), MODE_LEARNED>,
                             max_frames_per_wg<RATE, z_of_index(ZI)>(),
                             {&occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0, true>,
-                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_NOHOIST, true>},
+                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN, true>},
                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_LEARNED, 0, true>,
                             fb_launcher<RATE, z_of_index(ZI)>(),
                             fb_occupancy<RATE, z_of_index(ZI)>()}...}};

@@ -54,6 +54,19 @@ struct DecodeArgs {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// LDS byte address of a shared-memory pointer, and back: the address arithmetic
+// of the rotated row reads then stays in 32-bit LDS space, where a constant
+// term folds into the ds instruction's offset field.
+typedef __attribute__((address_space(3))) float lds_float;
+__device__ __forceinline__ uint32_t lds_addr(unsigned char* p)
+{
+    return (uint32_t)(uintptr_t)(__attribute__((address_space(3))) unsigned char*)p;
+}
+__device__ __forceinline__ lds_float* lds_ptr(uint32_t a)
+{
+    return (lds_float*)(uintptr_t)a;
+}
+
 constexpr int kLdsPad = 512;   // per-frame counters; also keeps wrapped row bases >= 0
 constexpr int kTileSlot = 504; // persistent mode: the tile index broadcast, inside the pad
 constexpr int kQueueOff = 128;      // MODE_LEARNED_FB: re-queue of failed frame ids (2F uint32)
@@ -73,7 +86,12 @@ enum { MODE_NMS = 0, MODE_LEARNED = 1, MODE_LEARNED_FB = 2 };
 //                distinct shift live across the iteration loop -> fewer VGPRs
 //   VAR_REREAD   reload rx and recompute L = rx - msg in the second pass
 //                instead of keeping the row's L values live -> fewer VGPRs
-enum { VAR_NOHOIST = 1, VAR_REREAD = 2 };
+//   VAR_SEL      with VAR_NOHOIST: select between the two opaque row bases
+//                (r and r - Z) per slot, shift folded into the LDS offset
+//                (compare + select, 2 VALU per slot instead of 3)
+enum { VAR_NOHOIST = 1, VAR_REREAD = 2, VAR_SEL = 4 };
+// the low-register code shape of the tuned variants
+constexpr int VAR_LEAN = VAR_NOHOIST | VAR_SEL;
 
 template <int RATE, int Z>
 constexpr int max_frames_per_wg()
@@ -181,9 +199,21 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         constexpr int D = CD::deg(L);
         int rr = r;
         if constexpr ((VAR & VAR_NOHOIST) != 0) asm volatile("" : "+v"(rr));
-        auto atl = [&](auto Lc2, auto Jc2) -> float* {
-            if constexpr ((VAR & VAR_NOHOIST) != 0) return at_r(Lc2, Jc2, rr);
-            else return at(Lc2, Jc2);
+        // VAR_SEL: LDS byte addresses of the two row bases, opaque per layer
+        uint32_t ob0 = lds_addr(smem) + (uint32_t)(kLdsPad + ((active ? f : 0) * N + r) * 4);
+        uint32_t obW = ob0 - (uint32_t)(Z * 4);
+        if constexpr ((VAR & VAR_SEL) != 0) asm volatile("" : "+v"(ob0), "+v"(obW));
+        auto atl = [&](auto Lc2, auto Jc2) {
+            if constexpr ((VAR & VAR_SEL) != 0) {
+                constexpr int b = CD::blk(L, decltype(Jc2)::value);
+                constexpr int s = CD::sh(L, decltype(Jc2)::value);
+                const uint32_t o = (s != 0 && rr >= Z - s) ? obW : ob0;
+                return lds_ptr(o + (uint32_t)((b * Z + s) * 4));
+            } else if constexpr ((VAR & VAR_NOHOIST) != 0) {
+                return at_r(Lc2, Jc2, rr);
+            } else {
+                return at(Lc2, Jc2);
+            }
         };
         float x[D];
         float mn1 = 10000.0f, mn2 = 10000.0f;
