@@ -67,6 +67,42 @@ __device__ __forceinline__ lds_float* lds_ptr(uint32_t a)
     return (lds_float*)(uintptr_t)a;
 }
 
+constexpr int gcd_int(int a, int b) { return b == 0 ? a : gcd_int(b, a % b); }
+
+// Lane masks of the wrapped row reads: wave w holds threads 64w .. 64w+63,
+// thread t owns row t mod Z, and slot (l, j) of that row wraps when
+// row >= Z - shift(l, j).  The pattern repeats every P = Z / gcd(64, Z) waves.
+template <int RATE, int Z>
+struct WrapMasks {
+    using CD = Code<RATE, Z>;
+    static constexpr int P = Z / gcd_int(64, Z);
+    struct Table {
+        uint64_t m[P][CD::NL][CD::C];
+    };
+    static constexpr Table make()
+    {
+        Table t{};
+        for (int p = 0; p < P; p++)
+            for (int l = 0; l < CD::NL; l++)
+                for (int j = 0; j < CD::C; j++) {
+                    uint64_t m = 0;
+                    if (j < CD::deg(l) && CD::sh(l, j) != 0)
+                        for (int lane = 0; lane < 64; lane++)
+                            if ((64 * p + lane) % Z >= Z - CD::sh(l, j)) m |= 1ull << lane;
+                    t.m[p][l][j] = m;
+                }
+        return t;
+    }
+    static constexpr Table table = make();
+};
+
+// this lane's bit of a wave-uniform mask set ? b : a (one v_cndmask on the
+// mask's scalar registers)
+__device__ __forceinline__ uint32_t select_by_mask(uint32_t a, uint32_t b, uint64_t mask)
+{
+    return __builtin_amdgcn_inverse_ballot_w64(mask) ? b : a;
+}
+
 constexpr int kLdsPad = 512;   // per-frame counters; also keeps wrapped row bases >= 0
 constexpr int kTileSlot = 504; // persistent mode: the tile index broadcast, inside the pad
 constexpr int kQueueOff = 128;      // MODE_LEARNED_FB: re-queue of failed frame ids (2F uint32)
@@ -89,9 +125,12 @@ enum { MODE_NMS = 0, MODE_LEARNED = 1, MODE_LEARNED_FB = 2 };
 //   VAR_SEL      with VAR_NOHOIST: select between the two opaque row bases
 //                (r and r - Z) per slot, shift folded into the LDS offset
 //                (compare + select, 2 VALU per slot instead of 3)
-enum { VAR_NOHOIST = 1, VAR_REREAD = 2, VAR_SEL = 4 };
+//   VAR_SMASK    with VAR_SEL: the per-slot "row wraps" lane masks come from a
+//                compile-time table (scalar loads) instead of a compare per
+//                slot, so the select is the only VALU op of the address
+enum { VAR_NOHOIST = 1, VAR_REREAD = 2, VAR_SEL = 4, VAR_SMASK = 8 };
 // the low-register code shape of the tuned variants
-constexpr int VAR_LEAN = VAR_NOHOIST | VAR_SEL;
+constexpr int VAR_LEAN = VAR_NOHOIST | VAR_SEL | VAR_SMASK;
 
 template <int RATE, int Z>
 constexpr int max_frames_per_wg()
@@ -125,6 +164,8 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     const int f = tid / Z;
     const int r = tid - f * Z;
     const bool active = f < Feff;
+    // wave phase of the wrap-mask table (uniform over the wave)
+    const int wphase = (__builtin_amdgcn_readfirstlane(tid) >> 6) % WrapMasks<RATE, Z>::P;
     // global frame of slot s, and its element offset
     auto gframe = [&](int s) -> unsigned long long {
         if constexpr (EXT) {
@@ -207,7 +248,14 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             if constexpr ((VAR & VAR_SEL) != 0) {
                 constexpr int b = CD::blk(L, decltype(Jc2)::value);
                 constexpr int s = CD::sh(L, decltype(Jc2)::value);
-                const uint32_t o = (s != 0 && rr >= Z - s) ? obW : ob0;
+                uint32_t o;
+                if constexpr (s == 0) {
+                    o = ob0;
+                } else if constexpr ((VAR & VAR_SMASK) != 0) {
+                    o = select_by_mask(ob0, obW, WrapMasks<RATE, Z>::table.m[wphase][L][decltype(Jc2)::value]);
+                } else {
+                    o = rr >= Z - s ? obW : ob0;
+                }
                 return lds_ptr(o + (uint32_t)((b * Z + s) * 4));
             } else if constexpr ((VAR & VAR_NOHOIST) != 0) {
                 return at_r(Lc2, Jc2, rr);
