@@ -265,7 +265,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         };
         float x[D];
         float mn1 = 10000.0f, mn2 = 10000.0f;
-        uint32_t sx = 0;
+        uint32_t sx = 0, sprev = 0;
         static_for<0, D>([&](auto Jc) {
             constexpr int J = decltype(Jc)::value;
             const float v = *atl(Lc, Jc) - msg[L][J];
@@ -273,7 +273,13 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             const float av = fabsf(v);
             mn2 = __builtin_amdgcn_fmed3f(mn1, mn2, av);
             mn1 = fminf(mn1, av);
-            sx ^= __float_as_uint(v);
+            // sign parity, two slots per three-input XOR
+            if constexpr ((J & 1) == 0) {
+                sprev = __float_as_uint(v);
+                if constexpr (J == D - 1) sx ^= sprev;
+            } else {
+                sx = __builtin_amdgcn_bitop3_b32(sx, sprev, __float_as_uint(v), 0x96);   // a ^ b ^ c
+            }
         });
         // message sign bit of slot n = signbit(L_n) ^ parity(signbits) ^ (D & 1)
         const uint32_t kw = (sx ^ ((uint32_t)(D & 1) << 31)) & 0x80000000u;
