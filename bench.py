@@ -95,7 +95,7 @@ def main():
 
     # HBM bytes and VALU issue of this kernel from the committed rocprofv3
     # counter runs (FETCH_SIZE x2 and WRITE_SIZE x1 after copy calibration)
-    traffic, valu = None, None
+    traffic, valu, lds = None, None, None
     pmc = os.path.join(ROOT, "profiles", "tdmp_r5z96_f65536_traffic.json")
     if os.path.exists(pmc):
         prof = json.load(open(pmc))
@@ -108,6 +108,12 @@ def main():
                     "issue_per_cu_cycle_at_2.4GHz": ipc,
                     "issue_frac_counters": prof.get("valu_issue_frac"),
                     "note": "the kernel is bound by VALU issue, not HBM"}
+        if prof.get("lds_bytes_per_launch_est"):
+            # LDS peak: 128 B/clk/CU x 256 CUs x 2.4 GHz
+            lds_tbs = prof["lds_bytes_per_launch_est"] / (kavg * 1e-3) / 1e12
+            lds = {"bytes_per_launch": prof["lds_bytes_per_launch_est"], "achieved_tbs": lds_tbs,
+                   "peak_tbs": 78.6, "frac": lds_tbs / 78.6,
+                   "bank_conflict_cycles": prof.get("lds_bank_conflict_cycles")}
 
     # The same decode through the host-pointer entry point (decode()'s
     # semantics: host LLRs in, posteriors back into them, 0/1 bytes out):
@@ -167,7 +173,7 @@ def main():
                    "frames": F, "frames_per_wg": dec.frames_per_wg, "iterations": ITERS},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel_ms": kavg, "bytes_per_frame": bytes_per_frame, "valu": valu},
+                     "kernel_ms": kavg, "bytes_per_frame": bytes_per_frame, "valu": valu, "lds": lds},
         "cpu_baseline": cpu,
         "pcie_inclusive": pcie,
         "info_ber": ber,

@@ -67,7 +67,7 @@ constexpr auto make_decode_table(std::integer_sequence<int, ZI...>)
     struct Arr { DecodeEntry e[sizeof...(ZI)]; };
     return Arr{{DecodeEntry{{&launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0>,
                              &launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN>},
-                            &launch_tdmp<RATE, z_of_index(ZI), MODE_LEARNED>,
+                            &launch_tdmp<RATE, z_of_index(ZI), MODE_LEARNED, 0>,
                             max_frames_per_wg<RATE, z_of_index(ZI)>(),
                             {&occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0, true>,
                              &occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN, true>},

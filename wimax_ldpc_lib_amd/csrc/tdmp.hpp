@@ -484,7 +484,7 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
             if (!last) {
                 const unsigned long long left = nfr - frame0;
                 const int fe = left < (unsigned long long)F ? (int)left : F;
-                tdmp_tile<RATE, Z, MODE_LEARNED, 0, true>(a, smem, frame0, fe, nullptr, fails);
+                tdmp_tile<RATE, Z, MODE_LEARNED, VAR, true>(a, smem, frame0, fe, nullptr, fails);
                 __syncthreads();
                 if (threadIdx.x == 0) {
                     const unsigned long long s0 = a.bucket ? (unsigned long long)a.bucket[0] : 0ull;
