@@ -64,6 +64,14 @@ namespace wimax_ldpc_lib {
              * GPU decode; returns the running BER. */
             double run_batch(double EbNo_dB, size_t nframes);
 
+            /* Extension: the loop "run_iteration() up to max_frames times,
+             * stop after the first frame with a bit error"
+             * (lib/Capacity_plot_example.cc:147-154), decoded in GPU batches.
+             * Leaves the frame source and the counters exactly where that
+             * loop would; returns true if an error occurred, and the number
+             * of frames the loop ran in *frames_run (nullable). */
+            bool run_until_error(double EbNo_dB, size_t max_frames, size_t* frames_run);
+
             uint64_t get_total_num_bits();
             uint64_t get_total_num_errors();
             unsigned int get_codeword_len();

@@ -14,6 +14,13 @@
 //       lib/BER_runner.cc:124-191 (LFSR data words, mt19937(0) +
 //       normal_distribution<float> samples) and writes <out>.data (uint8
 //       [F][k]) and <out>.noise (float [F][n]); prints one JSON line.
+//   drop_in_check stream <rate> <z> <nframes> <out_prefix>
+//       only the restated frame source: <out>.data and <out>.noise
+//   drop_in_check until <rate> <z> <ebn0> <max_frames> <repeats>
+//       the reference's "run_iteration until the first error" loop
+//       (lib/Capacity_plot_example.cc:147-156) against
+//       BER_runner::run_until_error, repeats times on one stream each;
+//       prints one JSON line with both sequences
 #include <math.h>
 #include <omp.h>
 #include <stdint.h>
@@ -98,6 +105,83 @@ static int frames_mode(char** a)
     return 0;
 }
 
+// the frame source of lib/BER_runner.cc:124-191, restated: LFSR data words
+// (x^31 + x^28 + 1, seed 0xdeadbeef, reset every 115200 bits) and the
+// mt19937(0) / normal_distribution<float> samples of each frame
+static void restated_source(unsigned n, unsigned k, unsigned F, std::vector<uint8_t>& data,
+                            std::vector<float>& noise)
+{
+    uint32_t lfsr = 0xdeadbeef;
+    unsigned lfsr_index = 0;
+    std::mt19937 gen(0);
+    std::normal_distribution<float> dist(0.0, 1.0);
+    data.assign((size_t)F * k, 0);
+    noise.assign((size_t)F * n, 0.0f);
+    for (unsigned f = 0; f < F; f++) {
+        for (unsigned i = 0; i < k; i++) {
+            const uint32_t bit = ((lfsr >> 0) ^ (lfsr >> 3)) & 1;
+            lfsr = (lfsr >> 1) | (bit << 30);
+            if (++lfsr_index == 1152 * 100) {
+                lfsr = 0xdeadbeef;
+                lfsr_index = 0;
+            }
+            data[(size_t)f * k + i] = (uint8_t)bit;
+        }
+        for (unsigned j = 0; j < n; j++) noise[(size_t)f * n + j] = dist(gen);
+    }
+}
+
+static int stream_mode(char** a)
+{
+    const coderate rate = (coderate)atoi(a[0]);
+    const unsigned z = atoi(a[1]), F = atoi(a[2]);
+    BER_runner r(rate, z, 1, 10);
+    std::vector<uint8_t> data;
+    std::vector<float> noise;
+    restated_source(r.get_codeword_len(), r.get_dataword_len(), F, data, noise);
+    const std::string out = a[3];
+    spit(out + ".data", data.data(), data.size());
+    spit(out + ".noise", noise.data(), noise.size() * 4);
+    printf("{\"n\": %u, \"k\": %u, \"frames\": %u}\n", r.get_codeword_len(), r.get_dataword_len(), F);
+    return 0;
+}
+
+static int until_mode(char** a)
+{
+    const coderate rate = (coderate)atoi(a[0]);
+    const unsigned z = atoi(a[1]);
+    const double ebno = atof(a[2]);
+    const unsigned max_frames = atoi(a[3]), repeats = atoi(a[4]);
+    BER_runner ra(rate, z, 1, 10), rb(rate, z, 1, 10);
+    std::string ja = "[", jb = "[";
+    for (unsigned r = 0; r < repeats; r++) {
+        unsigned j = 0;
+        double ber = 0.0;
+        for (j = 0; j < max_frames; j++) {
+            ber = ra.run_iteration(ebno);
+            if (ber != 0.0) break;
+        }
+        const unsigned fa = ber != 0.0 ? j + 1 : max_frames;
+        char buf[160];
+        snprintf(buf, sizeof buf, "%s[%u, %d, %llu, %llu]", r ? ", " : "", fa, ber != 0.0 ? 1 : 0,
+                 (unsigned long long)ra.get_total_num_bits(), (unsigned long long)ra.get_total_num_errors());
+        ja += buf;
+        ra.reset_test();
+        size_t fb = 0;
+        const bool err = rb.run_until_error(ebno, max_frames, &fb);
+        snprintf(buf, sizeof buf, "%s[%zu, %d, %llu, %llu]", r ? ", " : "", fb, err ? 1 : 0,
+                 (unsigned long long)rb.get_total_num_bits(), (unsigned long long)rb.get_total_num_errors());
+        jb += buf;
+        rb.reset_test();
+    }
+    // both streams continue from the same place: one more batch of frames
+    const double tail_a = (ra.run_batch(ebno, 64), (double)ra.get_total_num_errors());
+    const double tail_b = (rb.run_batch(ebno, 64), (double)rb.get_total_num_errors());
+    printf("{\"loop\": %s], \"until\": %s], \"tail_loop\": %.0f, \"tail_until\": %.0f}\n", ja.c_str(), jb.c_str(),
+           tail_a, tail_b);
+    return 0;
+}
+
 static int ber_mode(char** a)
 {
     const coderate rate = (coderate)atoi(a[0]);
@@ -114,26 +198,10 @@ static int ber_mode(char** a)
     rc.run_batch(ebn0, F / 3);
     rc.run_batch(ebn0, F - F / 3);
 
-    // the frame source of lib/BER_runner.cc, restated
+    std::vector<uint8_t> data;
+    std::vector<float> noise;
+    restated_source(ra.get_codeword_len(), ra.get_dataword_len(), F, data, noise);
     const unsigned n = ra.get_codeword_len(), k = ra.get_dataword_len();
-    uint32_t lfsr = 0xdeadbeef;
-    unsigned lfsr_index = 0;
-    std::mt19937 gen(0);
-    std::normal_distribution<float> dist(0.0, 1.0);
-    std::vector<uint8_t> data((size_t)F * k);
-    std::vector<float> noise((size_t)F * n);
-    for (unsigned f = 0; f < F; f++) {
-        for (unsigned i = 0; i < k; i++) {
-            const uint32_t bit = ((lfsr >> 0) ^ (lfsr >> 3)) & 1;   // x^31 + x^28 + 1
-            lfsr = (lfsr >> 1) | (bit << 30);
-            if (++lfsr_index == 1152 * 100) {
-                lfsr = 0xdeadbeef;
-                lfsr_index = 0;
-            }
-            data[(size_t)f * k + i] = (uint8_t)bit;
-        }
-        for (unsigned j = 0; j < n; j++) noise[(size_t)f * n + j] = dist(gen);
-    }
     spit(out + ".data", data.data(), data.size());
     spit(out + ".noise", noise.data(), noise.size() * 4);
     printf("{\"n\": %u, \"k\": %u, \"a_bits\": %llu, \"a_errors\": %llu, \"b_bits\": %llu, \"b_errors\": %llu, "
@@ -148,6 +216,8 @@ int main(int argc, char** argv)
 {
     if (argc == 8 && !strcmp(argv[1], "frames")) return frames_mode(argv + 2);
     if (argc == 8 && !strcmp(argv[1], "ber")) return ber_mode(argv + 2);
+    if (argc == 6 && !strcmp(argv[1], "stream")) return stream_mode(argv + 2);
+    if (argc == 7 && !strcmp(argv[1], "until")) return until_mode(argv + 2);
     fprintf(stderr, "usage: drop_in_check frames|ber ...\n");
     return 2;
 }

@@ -132,3 +132,96 @@ def test_ber_runner_matches_restated_frame_source(gpu, oracle_mod, tmp_path, rat
     errors = int((o["hd"].astype(np.uint8) != cw).sum())
     assert errors == r["a_errors"], (errors, r)
     assert 0 < errors < F * n // 20     # the operating point has some errors, not a broken channel
+
+
+def _restated_channel(code, rate, noise, cw, ebn0):
+    """lib/BER_runner.cc:170-191 with its operand types."""
+    R = {0: 0.5, 1: 2.0 / 3.0, 2: 2.0 / 3.0, 3: 0.75, 5: 5.0 / 6.0}[rate]
+    amp = np.float32((1.0 / np.sqrt(10.0 ** (ebn0 / 10.0))) * np.sqrt(1.0 / (np.float64(np.float32(R)) * 1)))
+    return (cw.astype(np.float64) * 2 - 1
+            + (np.sqrt(0.5) * np.float64(amp)) * noise.astype(np.float64)).astype(np.float32)
+
+
+@pytest.mark.parametrize("rate,z,ebn0,max_frames,repeats,clean_runs",
+                         [(0, 24, 2.5, 150, 6, True), (5, 48, 3.4, 120, 5, False)])
+def test_run_until_error_matches_reference_loop(gpu, rate, z, ebn0, max_frames, repeats, clean_runs):
+    """BER_runner::run_until_error (batched) stops where the reference's
+    frame-by-frame loop stops, with the same counters, and leaves the frame
+    stream at the same place."""
+    p = _run([CHECK, "until", str(rate), str(z), repr(ebn0), str(max_frames), str(repeats)])
+    assert p.returncode == 0, p.stdout + p.stderr
+    r = json.loads(p.stdout.strip().splitlines()[-1])
+    assert r["loop"] == r["until"], r
+    assert r["tail_loop"] == r["tail_until"]
+    flags = [x[1] for x in r["loop"]]
+    assert sum(flags) > 0, r                   # some runs stop at an error
+    if clean_runs:
+        assert sum(flags) < len(flags), r      # and some run clean to max_frames
+
+
+def test_ber_example_cli(gpu, oracle_mod, tmp_path):
+    """BER_example's printed BER per point equals the oracle's decode of the
+    restated frame stream with the reference's per-point sample sizes."""
+    import math
+    rate, z = 0, 24
+    cwd = os.getcwd()
+    os.chdir(str(tmp_path))
+    try:
+        p = _run([os.path.join(BIN, "BER_example"), "1", str(rate), str(z), "0", "2", "1"], timeout=300)
+    finally:
+        os.chdir(cwd)
+    assert p.returncode == 1, p.stdout + p.stderr
+    printed = [float(x) for x in re.findall(r"^BER: (\S+)$", p.stdout, re.M)]
+    points = [float(x) for x in re.findall(r"^EbN0\(dB\): (\S+)$", p.stdout, re.M)]
+    assert points == [0.0, 1.0, 2.0] and len(printed) == 3
+    code = oracle_mod.Code(rate, z)
+    n, k = code.n, code.k
+    F = 4000
+    q = _run([CHECK, "stream", str(rate), str(z), str(F), str(tmp_path / "s")])
+    assert q.returncode == 0, q.stdout + q.stderr
+    data = np.fromfile(str(tmp_path / "s.data"), dtype=np.uint8).reshape(F, k)
+    noise = np.fromfile(str(tmp_path / "s.noise"), dtype=np.float32).reshape(F, n)
+    cw = code.encode(data)
+    pos = 0
+
+    def errors(lo, hi, eb):
+        o = code.decode(_restated_channel(code, rate, noise[lo:hi], cw[lo:hi], eb), 10, True)
+        return int((o["hd"].astype(np.uint8) != cw[lo:hi]).sum())
+
+    for eb, ber_printed in zip(points, printed):
+        W = 46080 // n
+        e = errors(pos, pos + W, eb)
+        ber = e / (W * k)
+        num_req = math.floor(3.912 / 1e-8) if ber == 0.0 else int(10 * math.floor(3.912 / ber))
+        N2 = num_req // n
+        assert pos + W + N2 <= F
+        e += errors(pos + W, pos + W + N2, eb) if N2 else 0
+        ber = e / ((W + N2) * k)
+        assert "%e" % ber == "%e" % ber_printed, (eb, ber, ber_printed)
+        pos += W + N2
+    lines = open(str(tmp_path / "BER_LDPC_RATE0_N576_K288.csv")).read().strip().splitlines()
+    assert lines[0].startswith("ebno_db,ber") and len(lines) == 4
+    assert [float(l.split(",")[1]) for l in lines[1:]] == pytest.approx(printed, rel=1e-5)
+
+
+def test_capacity_example_cli(gpu, tmp_path):
+    """The Eb/N0 search at z = 24: one threshold per rate, ordered by rate,
+    above the BPSK-AWGN limit and within a few dB of it."""
+    cwd = os.getcwd()
+    os.chdir(str(tmp_path))
+    try:
+        p = _run([os.path.join(BIN, "Capacity_example"), "1", "24"], timeout=600)
+    finally:
+        os.chdir(cwd)
+    assert p.returncode == 1, p.stdout + p.stderr
+    found = re.findall(r"^Rate: (\S+), EbNo: (\S+)$", p.stdout, re.M)
+    assert [r for r, _ in found] == ["1/2", "2/3A", "2/3B", "3/4A", "5/6"]
+    lines = open(str(tmp_path / "CAP_LDPC_N576.csv")).read().strip().splitlines()
+    assert lines[0] == "rate,plot_rate,ebno_db,shannon_limit_db,bpsk_limit_db,frames_run" and len(lines) == 6
+    rows = [l.split(",") for l in lines[1:]]
+    eb = [float(r[2]) for r in rows]
+    for r, (name, printed) in zip(rows, found):
+        assert r[0] == name and abs(float(r[2]) - float(printed)) < 1e-6
+        shannon, bpsk = float(r[3]), float(r[4])
+        assert shannon < bpsk < float(r[2]) < bpsk + 6.0
+    assert eb[0] < eb[1] and eb[0] < eb[2] and max(eb[1], eb[2]) < eb[4] and eb[3] < eb[4]

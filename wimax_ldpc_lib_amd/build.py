@@ -42,6 +42,8 @@ BINDIR = os.path.join(PKG, "bin")
 PROGRAMS = [
     (os.path.join(PKG, "cli", "test_encoder.cpp"), os.path.join(BINDIR, "test_encoder"), "$ORIGIN/../lib"),
     (os.path.join(PKG, "cli", "test_decoder.cpp"), os.path.join(BINDIR, "test_decoder"), "$ORIGIN/../lib"),
+    (os.path.join(PKG, "cli", "BER_example.cpp"), os.path.join(BINDIR, "BER_example"), "$ORIGIN/../lib"),
+    (os.path.join(PKG, "cli", "Capacity_example.cpp"), os.path.join(BINDIR, "Capacity_example"), "$ORIGIN/../lib"),
     (os.path.join(ROOT, "tests", "cpp", "drop_in_check.cpp"), os.path.join(ROOT, "tests", "cpp", "_bin", "drop_in_check"),
      "$ORIGIN/../../../wimax_ldpc_lib_amd/lib"),
 ]

@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -290,6 +291,62 @@ double BER_runner::run_batch(double EbNo_dB, size_t nframes)
         m_num_errors += compare_data(cw.data() + f * n, dec.data() + f * n);
     }
     return (double)m_num_errors / (double)m_num_bits;
+}
+
+bool BER_runner::run_until_error(double EbNo_dB, size_t max_frames, size_t* frames_run)
+{
+    const size_t k = m_dataword_len, n = m_codeword_len;
+    size_t done = 0, chunk = 32;
+    while (done < max_frames) {
+        const size_t c = std::min(chunk, max_frames - done);
+        // frame source state before the chunk, to rewind to an error frame
+        const uint32_t lfsr = m_lfsr;
+        const unsigned lfsr_index = m_lfsr_index;
+        const std::mt19937 gen = m_generator_awgn;
+        const std::normal_distribution<float> dist = m_dist_awgn;
+        std::vector<uint8_t> data(c * k), cw(c * n);
+        std::vector<float> ch(c * n);
+        std::vector<int8_t> dec(c * n);
+        for (size_t f = 0; f < c; f++) fill_with_random(data.data() + f * k, (unsigned)k);
+        m_encoder->encode_batch(data.data(), c, cw.data());
+        for (size_t f = 0; f < c; f++) make_frame(EbNo_dB, data.data() + f * k, cw.data() + f * n, ch.data() + f * n);
+        m_decoder->decode_batch(ch.data(), c, dec.data(), nullptr, nullptr);
+        size_t first = c;
+        unsigned first_errors = 0;
+        for (size_t f = 0; f < c; f++) {
+            const unsigned e = compare_data(cw.data() + f * n, dec.data() + f * n);
+            if (e) {
+                first = f;
+                first_errors = e;
+                break;
+            }
+        }
+        if (first == c) {
+            m_num_bits += (uint64_t)c * k;
+            done += c;
+            chunk = std::min<size_t>(chunk * 2, 4096);
+            continue;
+        }
+        // rewind and consume exactly frames 0..first of the chunk
+        m_lfsr = lfsr;
+        m_lfsr_index = lfsr_index;
+        m_generator_awgn = gen;
+        m_dist_awgn = dist;
+        std::vector<uint8_t> d1(k), c1(n);
+        std::vector<float> ch1(n);
+        for (size_t f = 0; f <= first; f++) {
+            fill_with_random(d1.data(), (unsigned)k);
+            const uint8_t* cwf = cw.data() + f * n;   // same data word -> same codeword
+            std::copy(cwf, cwf + n, c1.begin());
+            make_frame(EbNo_dB, d1.data(), c1.data(), ch1.data());
+        }
+        m_num_bits += (uint64_t)(first + 1) * k;
+        m_num_errors += first_errors;
+        if (frames_run) *frames_run = done + first + 1;
+        return true;
+    }
+    if (frames_run) *frames_run = done;
+    return false;
 }
 
 }  // namespace wimax_ldpc_lib
