@@ -178,10 +178,26 @@ struct wl_decoder {
     // host-call staging
     DevBuf<float> d_llr;
     DevBuf<uint8_t> d_hard, d_ret, d_iters;
+    DevBuf<int8_t> d_dec;   // 0/1 bytes of the host-call decisions
     // fallback scratch
     DevBuf<uint32_t> d_idx, d_count;
     hipStream_t stream = nullptr;
 };
+
+// packed hard decisions [nframes][n/8] (bit v%8 of byte v/8) -> one 0/1 byte
+// per variable, the `decoded` format of ldpc_decoder::decode; four output
+// bytes per thread
+__global__ void unpack_hard_kernel(const uint8_t* __restrict__ hard, int8_t* __restrict__ out,
+                                   unsigned long long nbytes4)
+{
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < nbytes4; i += stride) {
+        // output bytes 4i .. 4i+3 are bits 4i%8 .. of packed byte 4i/8
+        const uint32_t b = hard[i >> 1] >> ((i & 1) * 4);
+        const uint32_t w = (b & 1u) | ((b >> 1) & 1u) << 8 | ((b >> 2) & 1u) << 16 | ((b >> 3) & 1u) << 24;
+        reinterpret_cast<uint32_t*>(out)[i] = w;
+    }
+}
 
 struct wl_encoder {
     HostCode code;
@@ -429,20 +445,22 @@ int wl_decode_f32_host(wl_decoder* dec, float* rx, size_t nframes, int8_t* decod
     const size_t nb = (size_t)n / 8;
     int rc;
     if ((rc = dec->d_llr.ensure(nframes * n)) || (rc = dec->d_hard.ensure(nframes * nb)) ||
-        (rc = dec->d_ret.ensure(nframes)) || (rc = dec->d_iters.ensure(nframes)))
+        (rc = dec->d_ret.ensure(nframes)) || (rc = dec->d_iters.ensure(nframes)) ||
+        (rc = dec->d_dec.ensure(nframes * n)))
         return rc;
     WL_HIP(hipMemcpy(dec->d_llr.p, rx, nframes * n * sizeof(float), hipMemcpyHostToDevice));
     rc = wl_decode_f32_dev(dec, dec->d_llr.p, nframes, dec->d_llr.p /* in place, as decode() */,
                            dec->d_hard.p, dec->d_ret.p, dec->d_iters.p, nullptr, nullptr);
     if (rc) return rc;
-    WL_HIP(hipDeviceSynchronize());
-    std::vector<uint8_t> hb(nframes * nb);
+    // n is a multiple of 8 (24 z, z a multiple of 4): whole packed bytes per frame
+    const unsigned long long n4 = (unsigned long long)nframes * n / 4;
+    const unsigned blocks = (unsigned)std::min<unsigned long long>((n4 + 255) / 256, 65536ull);
+    hipLaunchKernelGGL(unpack_hard_kernel, dim3(blocks), dim3(256), 0, nullptr, dec->d_hard.p, dec->d_dec.p, n4);
+    WL_HIP(hipGetLastError());
     WL_HIP(hipMemcpy(rx, dec->d_llr.p, nframes * n * sizeof(float), hipMemcpyDeviceToHost));
-    WL_HIP(hipMemcpy(hb.data(), dec->d_hard.p, hb.size(), hipMemcpyDeviceToHost));
+    WL_HIP(hipMemcpy(decoded, dec->d_dec.p, nframes * n, hipMemcpyDeviceToHost));
     WL_HIP(hipMemcpy(ret, dec->d_ret.p, nframes, hipMemcpyDeviceToHost));
     if (iters) WL_HIP(hipMemcpy(iters, dec->d_iters.p, nframes, hipMemcpyDeviceToHost));
-    for (size_t f = 0; f < nframes; f++)
-        for (int v = 0; v < n; v++) decoded[f * n + v] = (int8_t)((hb[f * nb + v / 8] >> (v % 8)) & 1);
     return WL_OK;
 }
 
