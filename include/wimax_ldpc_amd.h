@@ -262,6 +262,49 @@ int wl_channel_dev(wl_encoder* enc, uint64_t seed, uint64_t frame_offset, size_t
                    float ebn0_lo, float ebn0_hi, float* llr, uint8_t* bits, float* ebn0,
                    void* stream);
 
+/* ------------------------------------------------------- run-time QC codes */
+
+/* Quasi-cyclic LDPC codes given at run time, beyond the 114 compiled 802.16e
+ * tables: a base matrix of circulant shifts (802.11n-style tables, other
+ * lifting sizes) or an alist file in the format the reference writes and
+ * reads (python_ldpc/ldpc_utils.py:157-240, python_ldpc/ldpc_decoder.py:20-59).
+ * Row r of base row l checks variable b z + (r + base[l][b]) mod z for every
+ * block column b with base[l][b] >= 0.  Limits: mb <= 64 base rows, <= 32
+ * blocks per row, z <= 512, the frame's posteriors and messages within 64 KiB. */
+typedef struct wl_qc_code wl_qc_code;
+
+/* base: [mb][nb] shifts at lifting z, -1 = zero block. */
+int wl_qc_create(int nb, int mb, int z, const int16_t* base, wl_qc_code** out);
+
+/* text[0..len): an alist file.  z > 0: that lifting size (the rows must form
+ * z x z circulant blocks in layer-major order); z = 0: the largest one that
+ * fits (z = 1 for a code with no such structure: every row its own layer). */
+int wl_qc_from_alist(const char* text, size_t len, int z, wl_qc_code** out);
+int wl_qc_destroy(wl_qc_code* q);
+
+/* rate = -1; d and x are the dual-diagonal encoder constants, -1 when the
+ * parity part does not have that structure. */
+int wl_qc_params(const wl_qc_code* q, wl_code_info* out);
+int wl_qc_base(const wl_qc_code* q, int* nb, int* mb, int16_t* out, size_t out_len);
+
+/* expanded check-node table [m][c], ascending, -1 padded (the layout of
+ * wl_code_table) */
+int wl_qc_table(const wl_qc_code* q, int16_t* out, size_t out_len);
+
+/* the code as alist text (reference writer's format); *len gets the length
+ * without the terminating 0, out may be NULL to query it */
+int wl_qc_alist(const wl_qc_code* q, char* out, size_t cap, size_t* len);
+
+/* A decoder handle for the code: every wl_decode_* / wl_compute_syndrome_host
+ * entry point then decodes it with the reference's normalised min-sum
+ * (layered, one base row per layer), bit-exact as for the compiled codes.
+ * Learned offsets are not available for run-time codes. */
+int wl_decoder_create_qc(const wl_qc_code* q, int max_iter, unsigned flags, wl_decoder** out);
+
+/* An encoder handle (wl_encode_u8_*, wl_channel_dev) for a code with the
+ * dual-diagonal parity part (wl_qc_params: x >= 0); WL_ENOTSUP otherwise. */
+int wl_encoder_create_qc(const wl_qc_code* q, wl_encoder** out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,56 @@ int or_code_init(or_code* code, int rate, int z)
     return 0;
 }
 
+/* A quasi-cyclic code given by its base matrix at lifting z (shift -1 = zero
+ * block): the same expansion as or_code_init, rows layer-major, each row's
+ * variables ascending -- the table the reference's table-driven decoder walks
+ * (lib/ldpc_decoder.cc:143-240 reads only m_checknode_array and the row
+ * lengths).  rate = -1; d = x = -1 (no encoder constants). */
+int or_code_init_qc(or_code* code, int nb, int mb, int z, const int16_t* base)
+{
+    memset(code, 0, sizeof(*code));
+    if (nb < 2 || mb < 1 || mb >= nb || z < 1 || !base) return -1;
+    code->rate = -1;
+    code->z = z;
+    code->n = nb * z;
+    code->mb = mb;
+    code->m = mb * z;
+    code->k = code->n - code->m;
+    code->d = -1;
+    code->x = -1;
+    int c = 0;
+    code->deg = (int*)calloc(mb, sizeof(int));
+    for (int l = 0; l < mb; l++) {
+        int dg = 0;
+        for (int b = 0; b < nb; b++) dg += base[l * nb + b] >= 0;
+        code->deg[l] = dg;
+        if (dg > c) c = dg;
+    }
+    code->c = c;
+    code->blk = (int*)malloc(sizeof(int) * mb * c);
+    code->shift = (int*)malloc(sizeof(int) * mb * c);
+    code->cn = (int16_t*)malloc(sizeof(int16_t) * code->m * c);
+    code->row_len = (int*)malloc(sizeof(int) * code->m);
+    for (int l = 0; l < mb; l++) {
+        int j = 0;
+        for (int b = 0; b < nb; b++) {
+            if (base[l * nb + b] < 0) continue;
+            code->blk[l * c + j] = b;
+            code->shift[l * c + j] = base[l * nb + b];
+            j++;
+        }
+        for (; j < c; j++) { code->blk[l * c + j] = -1; code->shift[l * c + j] = -1; }
+        for (int r = 0; r < z; r++) {
+            int m = l * z + r;
+            for (j = 0; j < c; j++)
+                code->cn[m * c + j] = (j < code->deg[l])
+                    ? (int16_t)(code->blk[l * c + j] * z + (r + code->shift[l * c + j]) % z) : (int16_t)-1;
+            code->row_len[m] = code->deg[l];
+        }
+    }
+    return 0;
+}
+
 void or_code_free(or_code* code)
 {
     free(code->cn); free(code->row_len); free(code->blk); free(code->shift); free(code->deg);

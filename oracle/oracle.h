@@ -68,6 +68,8 @@ typedef struct {
 } or_dec_cfg;
 
 int  or_code_init(or_code* code, int rate, int z);
+/* quasi-cyclic code from a base matrix [mb][nb] of shifts at lifting z */
+int  or_code_init_qc(or_code* code, int nb, int mb, int z, const int16_t* base);
 void or_code_free(or_code* code);
 
 unsigned or_syndrome(const or_code* code, const int8_t* hd, int early_exit);

@@ -51,6 +51,7 @@ def lib():
         L = C.CDLL(LIB_PATH)
         P = C.c_void_p
         L.or_code_init.argtypes = [C.POINTER(_Code), C.c_int, C.c_int]
+        L.or_code_init_qc.argtypes = [C.POINTER(_Code), C.c_int, C.c_int, C.c_int, P]
         L.or_code_free.argtypes = [C.POINTER(_Code)]
         L.or_syndrome.argtypes = [C.POINTER(_Code), P, C.c_int]
         L.or_syndrome.restype = C.c_uint
@@ -76,12 +77,19 @@ def _p(a):
 class Code:
     """One (rate, z) code: sizes, the expanded [m][c] table and layer descriptors."""
 
-    def __init__(self, rate, z):
+    def __init__(self, rate, z, base=None):
+        """rate, z: a compiled 802.16e code; rate None with base: the
+        quasi-cyclic code of base matrix base [mb][nb] (shifts, -1 = zero) at z."""
         self._c = _Code()
-        rc = lib().or_code_init(C.byref(self._c), rate, z)
+        if base is not None:
+            base = np.ascontiguousarray(base, dtype=np.int16)
+            rc = lib().or_code_init_qc(C.byref(self._c), base.shape[1], base.shape[0], z, _p(base))
+        else:
+            rc = lib().or_code_init(C.byref(self._c), rate, z)
         if rc != 0:
-            raise ValueError("invalid rate/z %r/%r" % (rate, z))
+            raise ValueError("invalid code %r/%r" % (rate, z))
         c = self._c
+        rate = c.rate
         self.rate, self.z, self.n, self.m, self.k = rate, z, c.n, c.m, c.k
         self.c, self.mb, self.d, self.x = c.c, c.mb, c.d, c.x
         self.cn = np.ctypeslib.as_array(c.cn, shape=(c.m, c.c)).copy()

@@ -5,8 +5,8 @@ wimax_ldpc_lib_amd/build.py; this package only binds it.
 """
 from .capi import WLError, code_params, device_count, lib  # noqa: F401
 from .codec import (FIVESIXTHS, HALFRATE, THREEQUARTERSA, THREEQUARTERSB, TWOTHIRDSA,  # noqa: F401
-                    TWOTHIRDSB, ldpc, ldpc_decoder, ldpc_encoder, unpack_hard)
+                    TWOTHIRDSB, ldpc, ldpc_decoder, ldpc_encoder, qc_code, unpack_hard)
 
 __all__ = ["WLError", "code_params", "device_count", "lib", "ldpc", "ldpc_decoder", "ldpc_encoder",
-           "unpack_hard", "HALFRATE", "TWOTHIRDSA", "TWOTHIRDSB", "THREEQUARTERSA", "THREEQUARTERSB",
+           "qc_code", "unpack_hard", "HALFRATE", "TWOTHIRDSA", "TWOTHIRDSB", "THREEQUARTERSA", "THREEQUARTERSB",
            "FIVESIXTHS"]

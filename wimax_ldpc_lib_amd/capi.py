@@ -63,6 +63,15 @@ SIGNATURES = {
     "wl_encode_u8_dev": (C.c_int, [_P, _P, _SZ, _P, _P]),
     "wl_encode_u8_host": (C.c_int, [_P, _P, _SZ, _P]),
     "wl_channel_dev": (C.c_int, [_P, C.c_uint64, C.c_uint64, _SZ, C.c_float, C.c_float, _P, _P, _P, _P]),
+    "wl_qc_create": (C.c_int, [C.c_int, C.c_int, C.c_int, _P, C.POINTER(_P)]),
+    "wl_qc_from_alist": (C.c_int, [C.c_char_p, _SZ, C.c_int, C.POINTER(_P)]),
+    "wl_qc_destroy": (C.c_int, [_P]),
+    "wl_qc_params": (C.c_int, [_P, _P]),
+    "wl_qc_base": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), _P, _SZ]),
+    "wl_qc_table": (C.c_int, [_P, _P, _SZ]),
+    "wl_qc_alist": (C.c_int, [_P, _P, _SZ, C.POINTER(_SZ)]),
+    "wl_decoder_create_qc": (C.c_int, [_P, C.c_int, C.c_uint, C.POINTER(_P)]),
+    "wl_encoder_create_qc": (C.c_int, [_P, C.POINTER(_P)]),
 }
 
 

@@ -39,16 +39,77 @@ class ldpc:
     """Code parameters of (rate, z) -- include/ldpc.h:35-61."""
 
     def __init__(self, rate, z_factor, num_threads=1):
-        p = capi.code_params(int(rate), int(z_factor))
+        self._set_params(capi.code_params(int(rate), int(z_factor)))
+        self.num_threads = num_threads
+
+    def _set_params(self, p):
         self.rate, self.z = p["rate"], p["z"]
         self.n, self.m, self.k, self.c = p["n"], p["m"], p["k"], p["c"]
         self.mb, self.D, self.X = p["mb"], p["d"], p["x"]
-        self.num_threads = num_threads
 
     def checknode_array(self):
         t = np.zeros((self.m, self.c), dtype=np.int16)
-        check(lib().wl_code_table(self.rate, self.z, _ptr(t), t.size))
+        if getattr(self, "qc", None) is not None:
+            check(lib().wl_qc_table(self.qc._h, _ptr(t), t.size))
+        else:
+            check(lib().wl_code_table(self.rate, self.z, _ptr(t), t.size))
         return t
+
+
+class qc_code:
+    """A quasi-cyclic code given at run time: qc_code(base=[mb][nb] shifts,
+    z=lifting) or qc_code(alist=text, z=0 to detect the lifting size)."""
+
+    def __init__(self, base=None, z=None, alist=None):
+        h = C.c_void_p()
+        if alist is not None:
+            text = alist.encode() if isinstance(alist, str) else bytes(alist)
+            check(lib().wl_qc_from_alist(text, len(text), int(z or 0), C.byref(h)))
+        else:
+            b = np.ascontiguousarray(base, dtype=np.int16)
+            check(lib().wl_qc_create(b.shape[1], b.shape[0], int(z), _ptr(b), C.byref(h)))
+        self._h = h
+        ci = capi._CodeInfo()
+        check(lib().wl_qc_params(h, C.byref(ci)))
+        self.params = {f: getattr(ci, f) for f, _ in capi._CodeInfo._fields_}
+        self.z, self.n, self.m, self.k = ci.z, ci.n, ci.m, ci.k
+        self.mb, self.c, self.D, self.X = ci.mb, ci.c, ci.d, ci.x
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().wl_qc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def base(self):
+        nb, mb = C.c_int(), C.c_int()
+        check(lib().wl_qc_base(self._h, C.byref(nb), C.byref(mb), None, 0))
+        b = np.zeros((mb.value, nb.value), dtype=np.int16)
+        check(lib().wl_qc_base(self._h, None, None, _ptr(b), b.size))
+        return b
+
+    def checknode_array(self):
+        t = np.zeros((self.m, self.c), dtype=np.int16)
+        check(lib().wl_qc_table(self._h, _ptr(t), t.size))
+        return t
+
+    def alist(self):
+        n = C.c_size_t()
+        check(lib().wl_qc_alist(self._h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(lib().wl_qc_alist(self._h, buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
+
+    def decoder(self, max_iter, early_stop=True):
+        return ldpc_decoder.from_qc(self, max_iter, early_stop)
+
+    def encoder(self):
+        return ldpc_encoder.from_qc(self)
 
 
 class ldpc_decoder(ldpc):
@@ -62,6 +123,21 @@ class ldpc_decoder(ldpc):
         self._h = h
         self.max_iter = max_iter
         self.early_stop = early_stop
+
+    @classmethod
+    def from_qc(cls, qc, max_iter, early_stop=True):
+        """Decoder of a run-time quasi-cyclic code (qc_code)."""
+        self = cls.__new__(cls)
+        self._set_params(qc.params)
+        self.num_threads = 1
+        self.qc = qc
+        h = C.c_void_p()
+        flags = capi.WL_FLAG_EARLY_STOP if early_stop else 0
+        check(lib().wl_decoder_create_qc(qc._h, int(max_iter), flags, C.byref(h)))
+        self._h = h
+        self.max_iter = max_iter
+        self.early_stop = early_stop
+        return self
 
     def close(self):
         if getattr(self, "_h", None):
@@ -146,6 +222,18 @@ class ldpc_encoder(ldpc):
         h = C.c_void_p()
         check(lib().wl_encoder_create(int(rate), int(z_factor), C.byref(h)))
         self._h = h
+
+    @classmethod
+    def from_qc(cls, qc):
+        """Encoder of a run-time quasi-cyclic code with the dual-diagonal parity part."""
+        self = cls.__new__(cls)
+        self._set_params(qc.params)
+        self.num_threads = 1
+        self.qc = qc
+        h = C.c_void_p()
+        check(lib().wl_encoder_create_qc(qc._h, C.byref(h)))
+        self._h = h
+        return self
 
     def close(self):
         if getattr(self, "_h", None):

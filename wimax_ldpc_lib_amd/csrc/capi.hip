@@ -14,6 +14,8 @@
 #include "dispatch.hpp"
 #include "encoder.hpp"
 #include "errors.hpp"
+#include "generic.hpp"
+#include "qc_host.hpp"
 
 #define WL_VERSION_STR "wimax_ldpc_amd 0.1 (gfx950)"
 
@@ -179,6 +181,10 @@ struct wl_decoder {
     DevBuf<float> d_llr;
     DevBuf<uint8_t> d_hard, d_ret, d_iters;
     DevBuf<int8_t> d_dec;   // 0/1 bytes of the host-call decisions
+    // run-time quasi-cyclic code (wl_decoder_create_qc): descriptor on the device
+    bool generic = false;
+    wl::GenDesc gen{};
+    DevBuf<wl::GenDesc> d_gen;
     // fallback scratch
     DevBuf<uint32_t> d_idx, d_count;
     hipStream_t stream = nullptr;
@@ -196,6 +202,18 @@ __global__ void unpack_hard_kernel(const uint8_t* __restrict__ hard, int8_t* __r
         const uint32_t b = hard[i >> 1] >> ((i & 1) * 4);
         const uint32_t w = (b & 1u) | ((b >> 1) & 1u) << 8 | ((b >> 2) & 1u) << 16 | ((b >> 3) & 1u) << 24;
         reinterpret_cast<uint32_t*>(out)[i] = w;
+    }
+}
+
+// the same for any n: [nframes][ceil(n/8)] -> [nframes][n], one byte per thread
+__global__ void unpack_hard_any_kernel(const uint8_t* __restrict__ hard, int8_t* __restrict__ out, int n,
+                                       unsigned long long total)
+{
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    const unsigned long long nb8 = (unsigned long long)(n + 7) / 8;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const unsigned long long f = i / (unsigned long long)n, v = i - f * n;
+        out[i] = (int8_t)((hard[f * nb8 + v / 8] >> (v % 8)) & 1u);
     }
 }
 
@@ -292,6 +310,7 @@ int wl_tuned_launch(int rate, int z, int* frames_per_wg, int* variant)
 int wl_decoder_set_variant(wl_decoder* dec, int variant)
 {
     if (!dec) return fail(WL_EINVAL, "null decoder");
+    if (dec->generic) return variant == 0 ? WL_OK : fail(WL_EINVAL, "run-time codes have one code shape");
     if (variant < 0 || variant >= wl::kNumVariants)
         return fail(WL_EINVAL, "variant %d out of range 0..%d", variant, wl::kNumVariants - 1);
     dec->variant = variant;
@@ -308,6 +327,7 @@ int wl_decoder_get_variant(const wl_decoder* dec, int* out)
 int wl_decoder_set_frames_per_wg(wl_decoder* dec, int f)
 {
     if (!dec) return fail(WL_EINVAL, "null decoder");
+    if (dec->generic) return (f == 0 || f == 1) ? WL_OK : fail(WL_EINVAL, "run-time codes decode one frame per workgroup");
     if (f == 0) {
         int tv = 0;
         if (!tuned_lookup(dec->code.info.rate, dec->code.info.z, &f, &tv))
@@ -329,6 +349,7 @@ int wl_decoder_get_frames_per_wg(const wl_decoder* dec, int* out)
 int wl_decoder_set_offsets(wl_decoder* dec, const float* offsets_host, int T)
 {
     if (!dec) return fail(WL_EINVAL, "null decoder");
+    if (dec->generic && offsets_host) return fail(WL_ENOTSUP, "learned offsets need a compiled 802.16e code");
     if (!offsets_host) {
         dec->T = 0;
         return WL_OK;
@@ -347,6 +368,26 @@ static int decode_launch(wl_decoder* dec, bool learned, int max_iter, bool early
                          uint8_t* iters, uint16_t* synd, const uint32_t* idx, const uint32_t* bucket,
                          hipStream_t s)
 {
+    if (dec->generic) {
+        if (learned || idx || bucket) return fail(WL_ENOTSUP, "run-time codes decode with normalised min-sum only");
+        wl::GenArgs g{};
+        g.desc = dec->d_gen.p;
+        g.llr = llr;
+        g.post = post;
+        g.hard = hard;
+        g.ret = ret;
+        g.iters = iters;
+        g.synd = synd;
+        g.nframes = nframes;
+        g.max_iter = max_iter;
+        g.early_stop = early_stop ? 1 : 0;
+        g.n = dec->gen.n;
+        g.edges = dec->gen.edges;
+        g.z = dec->gen.z;
+        hipError_t e = wl::launch_generic(g, s);
+        if (e != hipSuccess) return hip_fail(e, "generic decoder kernel launch");
+        return WL_OK;
+    }
     wl::DecodeArgs a{};
     a.llr = llr;
     a.post = post;
@@ -388,6 +429,7 @@ int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nfr
     if (nframes == 0) return WL_OK;
     if (!llr || !ret) return fail(WL_EINVAL, "llr and ret are required");
     if (fallback_iters < 0 || fallback_iters > 255) return fail(WL_EINVAL, "fallback_iters out of range");
+    if (dec->generic) return fail(WL_ENOTSUP, "learned offsets need a compiled 802.16e code");
     if (nframes > 0xffffffffull) return fail(WL_EINVAL, "nframes %zu too large", nframes);
     hipStream_t s = (hipStream_t)stream;
     int rc;
@@ -442,7 +484,7 @@ int wl_decode_f32_host(wl_decoder* dec, float* rx, size_t nframes, int8_t* decod
     if (nframes == 0) return WL_OK;
     if (!rx || !decoded || !ret) return fail(WL_EINVAL, "null argument");
     const int n = dec->code.info.n;
-    const size_t nb = (size_t)n / 8;
+    const size_t nb = ((size_t)n + 7) / 8;
     int rc;
     if ((rc = dec->d_llr.ensure(nframes * n)) || (rc = dec->d_hard.ensure(nframes * nb)) ||
         (rc = dec->d_ret.ensure(nframes)) || (rc = dec->d_iters.ensure(nframes)) ||
@@ -452,10 +494,17 @@ int wl_decode_f32_host(wl_decoder* dec, float* rx, size_t nframes, int8_t* decod
     rc = wl_decode_f32_dev(dec, dec->d_llr.p, nframes, dec->d_llr.p /* in place, as decode() */,
                            dec->d_hard.p, dec->d_ret.p, dec->d_iters.p, nullptr, nullptr);
     if (rc) return rc;
-    // n is a multiple of 8 (24 z, z a multiple of 4): whole packed bytes per frame
-    const unsigned long long n4 = (unsigned long long)nframes * n / 4;
-    const unsigned blocks = (unsigned)std::min<unsigned long long>((n4 + 255) / 256, 65536ull);
-    hipLaunchKernelGGL(unpack_hard_kernel, dim3(blocks), dim3(256), 0, nullptr, dec->d_hard.p, dec->d_dec.p, n4);
+    if (n % 8 == 0) {
+        // whole packed bytes per frame (always for 802.16e: n = 24 z, z a multiple of 4)
+        const unsigned long long n4 = (unsigned long long)nframes * n / 4;
+        const unsigned blocks = (unsigned)std::min<unsigned long long>((n4 + 255) / 256, 65536ull);
+        hipLaunchKernelGGL(unpack_hard_kernel, dim3(blocks), dim3(256), 0, nullptr, dec->d_hard.p, dec->d_dec.p, n4);
+    } else {
+        const unsigned long long total = (unsigned long long)nframes * n;
+        const unsigned blocks = (unsigned)std::min<unsigned long long>((total + 255) / 256, 65536ull);
+        hipLaunchKernelGGL(unpack_hard_any_kernel, dim3(blocks), dim3(256), 0, nullptr, dec->d_hard.p, dec->d_dec.p,
+                           n, total);
+    }
     WL_HIP(hipGetLastError());
     WL_HIP(hipMemcpy(rx, dec->d_llr.p, nframes * n * sizeof(float), hipMemcpyDeviceToHost));
     WL_HIP(hipMemcpy(decoded, dec->d_dec.p, nframes * n, hipMemcpyDeviceToHost));
@@ -496,6 +545,71 @@ int wl_compute_syndrome_host(const wl_decoder* dec, const int8_t* hd, int exit_e
         }
     }
     *out = errors;
+    return WL_OK;
+}
+
+int wl_decoder_create_qc(const wl_qc_code* q, int max_iter, unsigned flags, wl_decoder** out)
+{
+    if (!out || !q) return fail(WL_EINVAL, "null argument");
+    *out = nullptr;
+    if (max_iter < 0 || max_iter > 255) return fail(WL_EINVAL, "max_iter %d out of range 0..255", max_iter);
+    wl_decoder* d = new (std::nothrow) wl_decoder();
+    if (!d) return fail(WL_ENOMEM, "out of host memory");
+    d->code.info = q->info;
+    d->code.table = q->table;
+    d->max_iter = max_iter;
+    d->flags = flags;
+    d->generic = true;
+    d->gen = q->desc;
+    d->frames_per_wg = 1;
+    d->variant = 0;
+    int rc = d->d_gen.ensure(1);
+    if (!rc) {
+        hipError_t e = hipMemcpy(d->d_gen.p, &d->gen, sizeof(wl::GenDesc), hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy");
+    }
+    if (rc) {
+        delete d;
+        return rc;
+    }
+    *out = d;
+    return WL_OK;
+}
+
+int wl_encoder_create_qc(const wl_qc_code* q, wl_encoder** out)
+{
+    if (!out || !q) return fail(WL_EINVAL, "null argument");
+    *out = nullptr;
+    const wl_code_info& ci = q->info;
+    if (ci.x < 0) return fail(WL_ENOTSUP, "the code has no dual-diagonal parity part");
+    if (q->mb > wl::kMaxLayers || ci.c > wl::kMaxDeg || q->z > 127 || q->nb > 127 || (ci.n % 2) != 0)
+        return fail(WL_ENOTSUP, "encoder limits: <= %d base rows, <= %d blocks per row, z <= 127, nb <= 127, even n",
+                    wl::kMaxLayers, wl::kMaxDeg);
+    const int F = std::max(1, 256 / q->z);
+    if ((size_t)F * (ci.n + q->z) > 65536) return fail(WL_ENOTSUP, "frame too large for the encoder");
+    wl_encoder* e = new (std::nothrow) wl_encoder();
+    if (!e) return fail(WL_ENOMEM, "out of host memory");
+    e->code.info = ci;
+    e->code.table = q->table;
+    wl::EncodeDesc& d = e->desc;
+    d = wl::EncodeDesc{};
+    d.z = q->z;
+    d.n = ci.n;
+    d.m = ci.m;
+    d.k = ci.k;
+    d.mb = q->mb;
+    d.kb = q->nb - q->mb;
+    d.x = ci.x;
+    d.d = ci.d;
+    d.frames_per_wg = F;
+    for (int l = 0; l < q->mb; l++) {
+        d.deg[l] = (int8_t)q->desc.deg[l];
+        for (int j = 0; j < q->desc.deg[l]; j++) {
+            d.blk[l][j] = (int8_t)q->desc.blk[l][j];
+            d.sh[l][j] = (int8_t)q->desc.sh[l][j];
+        }
+    }
+    *out = e;
     return WL_OK;
 }
 
