@@ -56,6 +56,21 @@ def test_tables_match_reference_build(oracle_mod):
             assert (t["d"], t["x"], t["n"], t["m"]) == (c.d, c.x, c.n, c.m)
 
 
+def test_qc_init_matches_wimax_tables(oracle_mod):
+    """The oracle's base-matrix constructor (used for run-time QC codes)
+    reproduces the pinned 802.16e tables from their base matrices."""
+    for rate in range(6):
+        for z in oracle_mod.Z_VALUES:
+            c = oracle_mod.Code(rate, z)
+            base = np.full((c.mb, 24), -1, np.int16)
+            for l in range(c.mb):
+                for j in range(c.deg[l]):
+                    base[l, c.blk[l, j]] = c.shift[l, j]
+            q = oracle_mod.Code(None, z, base=base)
+            assert (q.n, q.m, q.k, q.c, q.mb) == (c.n, c.m, c.k, c.c, c.mb)
+            assert np.array_equal(q.cn, c.cn) and np.array_equal(q.row_len, c.row_len)
+
+
 def test_table_invariants(oracle_mod):
     for rate in range(6):
         for z in (24, 52, 96):
