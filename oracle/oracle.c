@@ -174,8 +174,8 @@ static void slice(int8_t* hd, const float* rx, int n)
 /* One row of the layered sweep: lib/ldpc_decoder.cc:146-239.  rx is read
  * and written through the row's variable list cn[0..len), msg is the row's
  * message slot, beta the row's learned offsets (OR_LEARNED only). */
-static void row_update(int variant, const int16_t* cn, int len, float* rx, float* msg,
-                       const float* beta)
+static inline __attribute__((always_inline)) void row_update(int variant, const int16_t* cn, int len, float* rx,
+                                                            float* msg, const float* beta)
 {
     float LNM[OR_MAXDEG];
     int sign = 1;                       /* bool sign = true */
@@ -250,13 +250,29 @@ void or_row_update(int variant, int len, float* L_in_post_out, float* msg, const
 static void sweep(const or_code* code, const or_dec_cfg* cfg, float* rx, float* lmn, int it0)
 {
     const int c = code->c;
-    for (int m = 0; m < code->m; m++) {
-        const float* beta = NULL;
-        if (cfg->variant == OR_LEARNED) {
-            int t = it0 < cfg->n_offset_iters ? it0 : cfg->n_offset_iters - 1;
-            beta = cfg->offsets + ((size_t)t * code->mb + (size_t)(m / code->z)) * c;
+    /* the variant is a constant in each loop, so the row update is compiled
+     * once per variant, as the reference's #ifdef selects one at build time */
+    switch (cfg->variant) {
+    case OR_NMS:
+        for (int m = 0; m < code->m; m++)
+            row_update(OR_NMS, code->cn + (size_t)m * c, code->row_len[m], rx, lmn + (size_t)m * c, NULL);
+        break;
+    case OR_OMS:
+        for (int m = 0; m < code->m; m++)
+            row_update(OR_OMS, code->cn + (size_t)m * c, code->row_len[m], rx, lmn + (size_t)m * c, NULL);
+        break;
+    case OR_MS:
+        for (int m = 0; m < code->m; m++)
+            row_update(OR_MS, code->cn + (size_t)m * c, code->row_len[m], rx, lmn + (size_t)m * c, NULL);
+        break;
+    default: {
+        const int t = it0 < cfg->n_offset_iters ? it0 : cfg->n_offset_iters - 1;
+        for (int m = 0; m < code->m; m++) {
+            const float* beta = cfg->offsets + ((size_t)t * code->mb + (size_t)(m / code->z)) * c;
+            row_update(OR_LEARNED, code->cn + (size_t)m * c, code->row_len[m], rx, lmn + (size_t)m * c, beta);
         }
-        row_update(cfg->variant, code->cn + (size_t)m * c, code->row_len[m], rx, lmn + (size_t)m * c, beta);
+        break;
+    }
     }
 }
 

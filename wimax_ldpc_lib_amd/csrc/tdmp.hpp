@@ -326,18 +326,36 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
 
     // Failing parity checks among this lane's rows (lib/ldpc_decoder.cc:67-104
     // over hd = rx >= 0).
-    auto fail_rows = [&]() -> int {
+    // use_sel: address the rows like the layers do (one select per wrapped
+    // slot); used for the final check only -- inside the iteration loop that
+    // form raises the kernel's register count, so the early-stop check keeps
+    // the compare-and-select addressing
+    auto fail_rows = [&](auto selc) -> int {
+        constexpr bool use_sel = decltype(selc)::value;
         int fails = 0;
         static_for<0, NL>([&](auto Lc) {
             constexpr int L = decltype(Lc)::value;
             int rr = r;
             if constexpr ((VAR & VAR_NOHOIST) != 0) asm volatile("" : "+v"(rr));
+            const uint32_t ob0 = lds_addr(smem) + (uint32_t)(kLdsPad + ((active ? f : 0) * N + r) * 4);
+            const uint32_t obW = ob0 - (uint32_t)(Z * 4);
+            const uint64_t* mt = &WrapMasks<RATE, Z>::table.m[wphase][L][0];
+            if constexpr (use_sel && (VAR & VAR_SMASK) != 0) asm volatile("" : "+s"(mt));
             uint32_t p = 0;
             static_for<0, CD::deg(L)>([&](auto Jc) {
-                float* ad;
-                if constexpr ((VAR & VAR_NOHOIST) != 0) ad = at_r(Lc, Jc, rr);
-                else ad = at(Lc, Jc);
-                p ^= (*ad >= 0.0f) ? 1u : 0u;
+                constexpr int J = decltype(Jc)::value;
+                float xv;
+                if constexpr (use_sel && (VAR & VAR_SEL) != 0 && (VAR & VAR_SMASK) != 0) {
+                    constexpr int b = CD::blk(L, J);
+                    constexpr int s = CD::sh(L, J);
+                    const uint32_t o = s == 0 ? ob0 : select_by_mask(ob0, obW, mt[J]);
+                    xv = *lds_ptr(o + (uint32_t)((b * Z + s) * 4));
+                } else if constexpr ((VAR & VAR_NOHOIST) != 0) {
+                    xv = *at_r(Lc, Jc, rr);
+                } else {
+                    xv = *at(Lc, Jc);
+                }
+                p ^= (xv >= 0.0f) ? 1u : 0u;
             });
             fails += (int)p;
         });
@@ -348,7 +366,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     int it_done = 0;
     if (a.early_stop) {
         // initial hard decision + syndrome (lib/ldpc_decoder.cc:130-139)
-        if (active && fail_rows()) cnt[f] = 1;
+        if (active && fail_rows(std::false_type{})) cnt[f] = 1;
         __syncthreads();
         if (active && cnt[f] == 0) done = true;
         __syncthreads();
@@ -362,7 +380,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         });
         if (a.early_stop) {
             // lib/ldpc_decoder.cc:242-251
-            if (!done && fail_rows()) cnt[f] = 1;
+            if (!done && fail_rows(std::false_type{})) cnt[f] = 1;
             __syncthreads();
             if (!done && cnt[f] == 0) {
                 done = true;
@@ -378,7 +396,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     __syncthreads();
     if (a.synd || a.ret) {
         if (active) {
-            const int fr = fail_rows();
+            const int fr = fail_rows(std::true_type{});
             if (fr) atomicAdd(&cnt[f], (uint32_t)fr);
         }
     }
