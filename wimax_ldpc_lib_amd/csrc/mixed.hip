@@ -71,18 +71,37 @@ __global__ void mixed_scan_kernel(const uint32_t* __restrict__ counts, uint32_t*
     }
 }
 
+// Each workgroup takes a contiguous chunk of kScatterPer x 256 frames: ranks
+// within the chunk from LDS atomics, then one global reservation per code and
+// workgroup (global atomics on the 114 cursors from every frame contend
+// heavily).  The order of frames inside a bucket is immaterial: frames decode
+// independently.
+constexpr int kScatterPer = 8;
 __global__ __launch_bounds__(256) void mixed_scatter_kernel(const uint8_t* __restrict__ cfg, unsigned long long n,
                                                             const uint32_t* __restrict__ buckets,
                                                             uint32_t* __restrict__ cursors,
                                                             uint32_t* __restrict__ idx)
 {
-    for (unsigned long long g = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
-         g += (unsigned long long)gridDim.x * blockDim.x) {
-        const int c = cfg[g];
-        if (c >= kCodes) continue;
-        const uint32_t pos = atomicAdd(&cursors[c], 1u);
-        idx[buckets[2 * c] + pos] = (uint32_t)g;
+    __shared__ uint32_t h[kCodes];
+    __shared__ uint32_t base[kCodes];
+    for (int i = threadIdx.x; i < kCodes; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const unsigned long long g0 = (unsigned long long)blockIdx.x * (256 * kScatterPer);
+    int code[kScatterPer];
+    uint32_t rank[kScatterPer];
+#pragma unroll
+    for (int q = 0; q < kScatterPer; q++) {
+        const unsigned long long g = g0 + (unsigned long long)q * 256 + threadIdx.x;
+        code[q] = g < n ? (int)cfg[g] : kCodes;
+        rank[q] = code[q] < kCodes ? atomicAdd(&h[code[q]], 1u) : 0u;
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kCodes; i += blockDim.x)
+        base[i] = h[i] ? buckets[2 * i] + atomicAdd(&cursors[i], h[i]) : 0u;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kScatterPer; q++)
+        if (code[q] < kCodes) idx[base[code[q]] + rank[q]] = (uint32_t)(g0 + (unsigned long long)q * 256 + threadIdx.x);
 }
 
 // frame_off = exclusive prefix sum of n(cfg[g]); single pass per workgroup of
@@ -369,12 +388,27 @@ static int capture_decode(wl_mixed* m, int learned, int fallback_iters, const ui
     hipLaunchKernelGGL(mixed_prep_kernel, dim3(1), dim3(128), 0, s, m->counts, m->cursors, m->tiles);
     hipLaunchKernelGGL(mixed_hist_kernel, dim3(blocks), dim3(256), 0, s, cfg, n, m->counts);
     hipLaunchKernelGGL(mixed_scan_kernel, dim3(1), dim3(64), 0, s, m->counts, m->buckets);
-    hipLaunchKernelGGL(mixed_scatter_kernel, dim3(blocks), dim3(256), 0, s, cfg, n, m->buckets, m->cursors, m->idx);
+    const unsigned sblocks = (unsigned)((n + 256 * kScatterPer - 1) / (256 * kScatterPer));
+    hipLaunchKernelGGL(mixed_scatter_kernel, dim3(sblocks), dim3(256), 0, s, cfg, n, m->buckets, m->cursors, m->idx);
     e = hipEventRecord(m->ev_fork, s);
     for (int i = 0; i < kSideStreams && e == hipSuccess; i++) e = hipStreamWaitEvent(m->side[i], m->ev_fork, 0);
-    int launched = 0;
-    for (int c = 0; c < kCodes && e == hipSuccess; c++) {
-        if (!m->enabled[c]) continue;
+    // Longest first onto the least-loaded side stream, by edges per frame
+    // (z x nonzero base blocks): the per-code frame counts are only known on
+    // the device, so the estimate assumes codes of the batch equally likely.
+    int order[kCodes];
+    int norder = 0;
+    for (int c = 0; c < kCodes; c++)
+        if (m->enabled[c]) order[norder++] = c;
+    auto work = [](int c) {
+        const LayerTable lt = make_layer_table(c / kNumZ);
+        int nnz = 0;
+        for (int l = 0; l < lt.nl; l++) nnz += lt.deg[l];
+        return (long)(24 + 4 * (c % kNumZ)) * nnz;
+    };
+    std::stable_sort(order, order + norder, [&](int x, int y) { return work(x) > work(y); });
+    long load[kSideStreams] = {};
+    for (int oi = 0; oi < norder && e == hipSuccess; oi++) {
+        const int c = order[oi];
         DecodeArgs a{};
         a.llr = llr;
         a.post = post;
@@ -393,7 +427,9 @@ static int capture_decode(wl_mixed* m, int learned, int fallback_iters, const ui
         a.max_iter = m->max_iter;
         a.early_stop = (m->flags & WL_FLAG_EARLY_STOP) ? 1 : 0;
         const int rate = c / kNumZ;
-        hipStream_t ss = m->side[launched % kSideStreams];
+        const int si = (int)(std::min_element(load, load + kSideStreams) - load);
+        load[si] += work(c);
+        hipStream_t ss = m->side[si];
         if (learned && m->offsets[rate] && m->entries[c]->learned_fb) {
             a.offsets = m->offsets[rate];
             a.n_offset_iters = m->T[rate];
@@ -406,7 +442,6 @@ static int capture_decode(wl_mixed* m, int learned, int fallback_iters, const ui
         } else {
             e = m->entries[c]->nms[m->variant[c]](a, ss);
         }
-        launched++;
     }
     for (int i = 0; i < kSideStreams && e == hipSuccess; i++) {
         e = hipEventRecord(m->ev_join[i], m->side[i]);
