@@ -1,8 +1,5 @@
 """Measurements of the decoder workloads other than the headline bench.
 
-  small  rate 1/2, z = 24, 1024 frames at Eb/N0 2.5 dB, max 10 iterations with
-         the reference's early stop: GPU decode vs the CPU oracle on the same
-         frames (bit-exact check, timing)
   mixed  262144 frames, rate uniform over {1/2, 2/3A, 2/3B, 3/4A, 5/6}, z
          uniform over the 19 values, Eb/N0 uniform in [1.0, 4.5] dB, max 10
          iterations with early stop, decoded as one mixed batch (one captured
@@ -15,7 +12,10 @@
          min-sum (10 iterations) on the same frames: info Gbit/s, fallback
          fraction, BER / FER per rate
 
-    python -m wimax_ldpc_lib_amd.workloads small|mixed|fallback [--out file.json]
+    python -m wimax_ldpc_lib_amd.workloads mixed|fallback [--out file.json]
+
+(the small rate 1/2, z = 24 workload, measured against the CPU decoder, is
+bench.py --workload small)
 """
 import argparse
 import json
@@ -43,31 +43,6 @@ def timed(torch, fn, reps):
     e.record()
     torch.cuda.synchronize()
     return s.elapsed_time(e) / reps
-
-
-def small(a, torch, W, np):
-    from oracle import oracle as O
-    code = O.Code(0, 24)
-    enc = W.ldpc_encoder(0, 24)
-    F = 1024
-    llr = torch.empty((F, code.n), dtype=torch.float32, device="cuda")
-    enc.channel_dev(a.seed, 0, F, 2.5, llr=llr)
-    dec = W.ldpc_decoder(0, 24, 10, early_stop=True)
-    post = torch.empty_like(llr)
-    hard = torch.empty((F, code.n // 8), dtype=torch.uint8, device="cuda")
-    ret = torch.empty(F, dtype=torch.uint8, device="cuda")
-    iters = torch.empty(F, dtype=torch.uint8, device="cuda")
-    ms = timed(torch, lambda: dec.decode_dev(llr, post, hard, ret, iters), 20)
-    o = code.decode(llr.cpu().numpy(), 10, True, nthreads=O.omp_max_threads())
-    same = (np.array_equal(post.cpu().numpy().view(np.uint32), o["post"].view(np.uint32)) and
-            np.array_equal(hard.cpu().numpy(), np.packbits(o["hd"], axis=1, bitorder="little")) and
-            np.array_equal(iters.cpu().numpy(), o["iters"]) and np.array_equal(ret.cpu().numpy(), o["ret"]))
-    out = {"frames": F, "gpu_ms": ms, "gpu_info_gbps": F * code.k / (ms * 1e-3) / 1e9,
-           "cpu_seconds": o["seconds"], "cpu_threads": O.omp_max_threads(),
-           "cpu_info_gbps": F * code.k / o["seconds"] / 1e9, "bit_exact": bool(same),
-           "iters_histogram": np.bincount(o["iters"], minlength=11).tolist(), "failures": int(o["ret"].sum())}
-    log(json.dumps(out))
-    return out
 
 
 def mixed_inputs(torch, md, np, F, codes, seed, lo, hi):
@@ -204,7 +179,7 @@ def fallback(a, torch, W, np):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["small", "mixed", "fallback"])
+    ap.add_argument("which", choices=["mixed", "fallback"])
     ap.add_argument("--frames", type=int, default=0)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--seed", type=int, default=31337)
@@ -216,7 +191,7 @@ def main():
     import torch
 
     import wimax_ldpc_lib_amd as W
-    out = {"small": small, "mixed": mixed, "fallback": fallback}[a.which](a, torch, W, np)
+    out = {"mixed": mixed, "fallback": fallback}[a.which](a, torch, W, np)
     if a.out:
         json.dump(out, open(a.out, "w"), indent=1)
 
