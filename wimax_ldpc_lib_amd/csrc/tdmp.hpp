@@ -189,7 +189,17 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         const f32x4* src = reinterpret_cast<const f32x4*>(a.llr + frame0 * N);
         f32x4* dst = reinterpret_cast<f32x4*>(rxall);
         const int total4 = Feff * (N / 4);
-        for (int i = tid; i < total4; i += blockDim.x) dst[i] = __builtin_nontemporal_load(src + i);
+        const int bd = blockDim.x;
+        // up to 8 loads per thread in flight before the LDS stores
+        for (int i0 = tid; i0 < total4; i0 += 8 * bd) {
+            f32x4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (i0 + u * bd < total4) v[u] = __builtin_nontemporal_load(src + i0 + u * bd);
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (i0 + u * bd < total4) dst[i0 + u * bd] = v[u];
+        }
     } else {
         for (int s = 0; s < Feff; s++) {
             const f32x4* src = reinterpret_cast<const f32x4*>(a.llr + goff(s));
@@ -265,14 +275,38 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         };
         float x[D];
         float mn1 = 10000.0f, mn2 = 10000.0f;
+        float a0 = 0.0f, a1 = 0.0f;   // |L| of the first two slots of the current triple
         uint32_t sx = 0, sprev = 0;
         static_for<0, D>([&](auto Jc) {
             constexpr int J = decltype(Jc)::value;
             const float v = *atl(Lc, Jc) - msg[L][J];
             if constexpr ((VAR & VAR_REREAD) == 0) x[J] = v;
             const float av = fabsf(v);
-            mn2 = __builtin_amdgcn_fmed3f(mn1, mn2, av);
-            mn1 = fminf(mn1, av);
+            // min1 / min2 = the two smallest |L| of the row (a tie at the
+            // minimum gives min2 = min1, as the reference's rescan that skips
+            // only the first argmin does), clamped at 10000.  Whole triples:
+            // min3 / med3 give their two smallest, merged into the running
+            // pair in three operations; the last D % 3 slots go one at a time.
+            constexpr int Dt = D - D % 3;
+            if constexpr (J < Dt) {
+                if constexpr (J % 3 == 0) {
+                    a0 = av;
+                } else if constexpr (J % 3 == 1) {
+                    a1 = av;
+                } else {
+                    const float lo = fminf(fminf(a0, a1), av);
+                    const float mid = __builtin_amdgcn_fmed3f(a0, a1, av);
+                    if constexpr (J == 2) {
+                        mn2 = fminf(mn2, mid);
+                    } else {
+                        mn2 = fminf(fminf(mn2, fmaxf(mn1, lo)), mid);
+                    }
+                    mn1 = fminf(mn1, lo);
+                }
+            } else {
+                mn2 = __builtin_amdgcn_fmed3f(mn1, mn2, av);
+                mn1 = fminf(mn1, av);
+            }
             // sign parity, two slots per three-input XOR
             if constexpr ((J & 1) == 0) {
                 sprev = __float_as_uint(v);
