@@ -47,6 +47,7 @@ extern "C" {
 
 /* decoder flags */
 #define WL_FLAG_EARLY_STOP  0x1u  /* syndrome check before and after every iteration (as shipped) */
+#define WL_FLAG_MIXED_UNORDERED 0x2u  /* wl_mixed: no reliability order inside code buckets */
 
 typedef struct wl_decoder wl_decoder;
 typedef struct wl_encoder wl_encoder;
@@ -159,9 +160,13 @@ int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nfr
  * and ret / iters / synd to index g.  A decode is one hipGraph launch: frames
  * are bucketed by code on the device and each code's bucket is decoded by a
  * persistent kernel with that code's autotuned launch shape; results equal a
- * per-code wl_decode_f32_dev bit for bit.  The graph is captured at the first
- * call with a given set of buffers and replayed afterwards.  Frames whose code
- * id is invalid (or not enabled, see wl_mixed_set_codes) are skipped. */
+ * per-code wl_decode_f32_dev bit for bit.  With early stop (create flags),
+ * frames inside a bucket are arranged by a reliability estimate (mean |LLR|
+ * of the first 256 LLRs) so that the frames one workgroup decodes together
+ * tend to stop together; WL_FLAG_MIXED_UNORDERED turns that off (same
+ * results).  The graph is captured at the first call with a given set of
+ * buffers and replayed afterwards.  Frames whose code id is invalid (or not
+ * enabled, see wl_mixed_set_codes) are skipped. */
 typedef struct wl_mixed wl_mixed;
 int wl_mixed_create(int max_iter, unsigned flags, wl_mixed** out);
 int wl_mixed_destroy(wl_mixed* m);

@@ -97,3 +97,15 @@ def test_null_arguments_are_rejected_without_device_work():
     assert L.wl_decode_learned_fallback_dev(dec._h, None, 4, 10, None, None, None, None, None, None,
                                             None) == capi.WL_EINVAL   # no offsets set
     assert L.wl_code_params(0, 24, None) == capi.WL_EINVAL
+
+
+def test_header_flag_values_match_bindings():
+    """The #define values of the C header equal the constants the Python
+    bindings pass."""
+    from wimax_ldpc_lib_amd import capi
+    text = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                             "wimax_ldpc_amd.h")).read()
+    defs = dict(re.findall(r"#define\s+(WL_FLAG_\w+)\s+(0x[0-9a-fA-F]+)u", text))
+    assert defs, "no flags found"
+    for name, val in defs.items():
+        assert getattr(capi, name) == int(val, 16), name

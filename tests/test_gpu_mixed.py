@@ -133,3 +133,43 @@ def test_mixed_invalid_and_disabled_codes_are_skipped(gpu):
     p = post.cpu().numpy()
     g = int(np.flatnonzero(skipped)[0])
     assert (p[offs[g]:offs[g] + int(n_of[g])] == 7.0).all()
+
+
+def test_mixed_reliability_order_leaves_results_unchanged(gpu):
+    """Frames arranged by reliability inside the code buckets (the early-stop
+    default) decode exactly as without the arrangement, including frames whose
+    reliability estimate falls off either end of its range (all-zero LLRs,
+    tiny and huge magnitudes, a NaN), and those frames equal a per-code
+    decode."""
+    import wimax_ldpc_lib_amd as W
+    from wimax_ldpc_lib_amd.mixed import MixedDecoder
+    torch = gpu
+    codes = [0 * 19 + 0, 1 * 19 + 9, 2 * 19 + 3, 3 * 19 + 12, 5 * 19 + 18]
+    md, cfg_np, cfg, off, offs, n_of, llr, bits = _setup(torch, 4000, 41, codes)
+    x = llr.cpu().numpy()
+    edits = {0: 0.0, 1: 1e-6, 2: 1e30}
+    for g, scale in edits.items():
+        o, n = offs[g], int(n_of[g])
+        x[o:o + n] *= np.float32(scale)
+    llr = torch.from_numpy(x).cuda()
+    got = [t.cpu().numpy() for t in _mixed_decode(torch, md, cfg, off, llr)]
+    plain = MixedDecoder(max_iter=10, early_stop=True, ordered=False)
+    want = [t.cpu().numpy() for t in _mixed_decode(torch, plain, cfg, off, llr)]
+    for a, b in zip(got, want):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    post, hard, ret, iters, synd = got
+    for g in edits:
+        c = int(cfg_np[g])
+        rate, z = c // 19, 24 + 4 * (c % 19)
+        n, o = 24 * z, offs[g]
+        dec = W.ldpc_decoder(rate, z, 10, early_stop=True)
+        d = torch.from_numpy(np.ascontiguousarray(x[o:o + n]).reshape(1, n)).cuda()
+        p1 = torch.empty_like(d)
+        h1 = torch.empty((1, n // 8), dtype=torch.uint8, device="cuda")
+        r1 = torch.empty(1, dtype=torch.uint8, device="cuda")
+        i1 = torch.empty(1, dtype=torch.uint8, device="cuda")
+        dec.decode_dev(d, p1, h1, r1, i1)
+        torch.cuda.synchronize()
+        assert np.array_equal(post[o:o + n].view(np.uint32), p1.cpu().numpy()[0].view(np.uint32)), g
+        assert np.array_equal(hard[o // 8:(o + n) // 8], h1.cpu().numpy()[0]), g
+        assert ret[g] == r1.cpu().numpy()[0] and iters[g] == i1.cpu().numpy()[0], g

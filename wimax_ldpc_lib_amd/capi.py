@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libwimax_ldpc_amd.so")
 
 WL_OK, WL_EINVAL, WL_ENOMEM, WL_EDEVICE, WL_ENOTSUP = 0, -22, -12, -5, -95
 WL_FLAG_EARLY_STOP = 0x1
+WL_FLAG_MIXED_UNORDERED = 0x2
 
 # every function the header declares: name -> (restype, argtypes)
 _P = C.c_void_p

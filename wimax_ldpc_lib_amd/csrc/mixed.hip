@@ -4,10 +4,16 @@
 // holds n_g = 24 z_g LLRs starting at element frame_off[g] of a ragged buffer.
 // One decode is a hipGraph, captured once per set of buffers and replayed:
 //   prep      zero the per-code counters and persistent tile counters
+//   level     (early stop) a reliability level per frame from the mean |LLR|
+//             of its first 256 LLRs, and a counting sort of the frame ids by
+//             level (hist / scan / scatter below, over 256 levels)
 //   hist      frames per code (workgroup histogram in LDS, then one atomic per
 //             code per workgroup); invalid ids counted separately
 //   scan      bucket start of each code (exclusive scan, one workgroup)
-//   scatter   frame ids into their code's bucket
+//   scatter   frame ids into their code's bucket, in level order: a bucket is
+//             a sequence of short runs of frames of similar reliability, so the
+//             frames of one workgroup tile tend to stop after similar numbers
+//             of iterations (a workgroup runs until its slowest frame stops)
 //   decode    one persistent kernel per code (graph siblings on 4 streams),
 //             each with its autotuned frames per workgroup and code shape,
 //             taking tiles of its bucket from an atomic counter; an empty
@@ -33,37 +39,81 @@ namespace wl {
 constexpr int kCodes = kNumRates * kNumZ;   // 114
 constexpr int kSideStreams = 4;
 
-__global__ void mixed_prep_kernel(uint32_t* counts, uint32_t* cursors, unsigned long long* tiles)
+constexpr int kLevels = 256;
+constexpr int kLevelSample = 256;   // LLRs per frame in the reliability estimate (n >= 576)
+
+__global__ void mixed_prep_kernel(uint32_t* counts, uint32_t* cursors, uint32_t* lv_counts, uint32_t* lv_cursors,
+                                  unsigned long long* tiles)
 {
     for (int i = threadIdx.x; i <= kCodes; i += blockDim.x) {
         counts[i] = 0;
         cursors[i] = 0;
     }
+    for (int i = threadIdx.x; i <= kLevels; i += blockDim.x) {
+        lv_counts[i] = 0;
+        lv_cursors[i] = 0;
+    }
     for (int i = threadIdx.x; i < kCodes; i += blockDim.x) tiles[i] = 0;
 }
 
-__global__ __launch_bounds__(256) void mixed_hist_kernel(const uint8_t* __restrict__ cfg, unsigned long long n,
-                                                         uint32_t* __restrict__ counts)
+// Reliability level of frame g: the mean |LLR| of its first kLevelSample LLRs
+// as float bits >> 19 (exponent and 4 mantissa bits: 16 steps per octave),
+// over 2^-8 .. 2^8; lower means, zero and NaN give 0, higher ones 255.  One
+// wave per frame.  Only the order of frames within a code bucket depends on it.
+__global__ __launch_bounds__(256) void mixed_level_kernel(const uint8_t* __restrict__ cfg,
+                                                          const unsigned long long* __restrict__ off,
+                                                          const float* __restrict__ llr, unsigned long long n,
+                                                          uint8_t* __restrict__ level)
 {
-    __shared__ uint32_t h[kCodes + 1];
-    for (int i = threadIdx.x; i <= kCodes; i += blockDim.x) h[i] = 0;
+    const int lane = threadIdx.x & 63;
+    for (unsigned long long g = (unsigned long long)blockIdx.x * 4 + (threadIdx.x >> 6); g < n;
+         g += (unsigned long long)gridDim.x * 4) {
+        float sum = 0.0f;
+        if (cfg[g] < kCodes) {
+            const float* x = llr + off[g];
+#pragma unroll
+            for (int q = 0; q < kLevelSample / 64; q++) sum += fabsf(x[q * 64 + lane]);
+        }
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+        if (lane == 0) {
+            const float mean = sum * (1.0f / kLevelSample);
+            int lv = 0;
+            if (mean > 0.0f) {
+                const int k = (int)(__float_as_uint(mean) >> 19) - (127 - 8) * 16;
+                lv = k < 0 ? 0 : (k >= kLevels ? kLevels - 1 : k);
+            }
+            level[g] = (uint8_t)lv;
+        }
+    }
+}
+
+// Counting sort by a byte key with K buckets (keys >= K: counted in bin K,
+// not scattered).  hist: per-key counts (workgroup histogram in LDS, then one
+// atomic per key per workgroup).
+template <int K>
+__global__ __launch_bounds__(256) void bucket_hist_kernel(const uint8_t* __restrict__ key, unsigned long long n,
+                                                          uint32_t* __restrict__ counts)
+{
+    __shared__ uint32_t h[K + 1];
+    for (int i = threadIdx.x; i <= K; i += blockDim.x) h[i] = 0;
     __syncthreads();
     for (unsigned long long g = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
          g += (unsigned long long)gridDim.x * blockDim.x) {
-        const int c = cfg[g];
-        atomicAdd(&h[c < kCodes ? c : kCodes], 1u);
+        const int c = key[g];
+        atomicAdd(&h[c < K ? c : K], 1u);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i <= kCodes; i += blockDim.x)
+    for (int i = threadIdx.x; i <= K; i += blockDim.x)
         if (h[i]) atomicAdd(&counts[i], h[i]);
 }
 
-// buckets[c] = {start, count}
-__global__ void mixed_scan_kernel(const uint32_t* __restrict__ counts, uint32_t* __restrict__ buckets)
+// buckets[k] = {start, count}
+template <int K>
+__global__ void bucket_scan_kernel(const uint32_t* __restrict__ counts, uint32_t* __restrict__ buckets)
 {
     if (threadIdx.x == 0) {
         uint32_t s = 0;
-        for (int c = 0; c < kCodes; c++) {
+        for (int c = 0; c < K; c++) {
             buckets[2 * c] = s;
             buckets[2 * c + 1] = counts[c];
             s += counts[c];
@@ -71,37 +121,42 @@ __global__ void mixed_scan_kernel(const uint32_t* __restrict__ counts, uint32_t*
     }
 }
 
-// Each workgroup takes a contiguous chunk of kScatterPer x 256 frames: ranks
-// within the chunk from LDS atomics, then one global reservation per code and
-// workgroup (global atomics on the 114 cursors from every frame contend
-// heavily).  The order of frames inside a bucket is immaterial: frames decode
-// independently.
+// Element i of the input order (order[i], or i without one) goes to its key's
+// bucket.  Each workgroup takes a contiguous chunk of kScatterPer x 256
+// elements: ranks within the chunk from LDS atomics, then one global
+// reservation per key and workgroup (global atomics on the cursors from every
+// element contend heavily).  Order inside a chunk's run is arbitrary; frames
+// decode independently, so results do not depend on it.
 constexpr int kScatterPer = 8;
-__global__ __launch_bounds__(256) void mixed_scatter_kernel(const uint8_t* __restrict__ cfg, unsigned long long n,
-                                                            const uint32_t* __restrict__ buckets,
-                                                            uint32_t* __restrict__ cursors,
-                                                            uint32_t* __restrict__ idx)
+template <int K>
+__global__ __launch_bounds__(256) void bucket_scatter_kernel(const uint8_t* __restrict__ key,
+                                                             const uint32_t* __restrict__ order,
+                                                             unsigned long long n,
+                                                             const uint32_t* __restrict__ buckets,
+                                                             uint32_t* __restrict__ cursors,
+                                                             uint32_t* __restrict__ out)
 {
-    __shared__ uint32_t h[kCodes];
-    __shared__ uint32_t base[kCodes];
-    for (int i = threadIdx.x; i < kCodes; i += blockDim.x) h[i] = 0;
+    __shared__ uint32_t h[K];
+    __shared__ uint32_t base[K];
+    for (int i = threadIdx.x; i < K; i += blockDim.x) h[i] = 0;
     __syncthreads();
-    const unsigned long long g0 = (unsigned long long)blockIdx.x * (256 * kScatterPer);
+    const unsigned long long i0 = (unsigned long long)blockIdx.x * (256 * kScatterPer);
     int code[kScatterPer];
-    uint32_t rank[kScatterPer];
+    uint32_t rank[kScatterPer], gid[kScatterPer];
 #pragma unroll
     for (int q = 0; q < kScatterPer; q++) {
-        const unsigned long long g = g0 + (unsigned long long)q * 256 + threadIdx.x;
-        code[q] = g < n ? (int)cfg[g] : kCodes;
-        rank[q] = code[q] < kCodes ? atomicAdd(&h[code[q]], 1u) : 0u;
+        const unsigned long long i = i0 + (unsigned long long)q * 256 + threadIdx.x;
+        gid[q] = i < n ? (order ? order[i] : (uint32_t)i) : 0u;
+        code[q] = i < n ? (int)key[gid[q]] : K;
+        rank[q] = code[q] < K ? atomicAdd(&h[code[q]], 1u) : 0u;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kCodes; i += blockDim.x)
+    for (int i = threadIdx.x; i < K; i += blockDim.x)
         base[i] = h[i] ? buckets[2 * i] + atomicAdd(&cursors[i], h[i]) : 0u;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kScatterPer; q++)
-        if (code[q] < kCodes) idx[base[code[q]] + rank[q]] = (uint32_t)(g0 + (unsigned long long)q * 256 + threadIdx.x);
+        if (code[q] < K) out[base[code[q]] + rank[q]] = gid[q];
 }
 
 // frame_off = exclusive prefix sum of n(cfg[g]); single pass per workgroup of
@@ -179,6 +234,8 @@ struct wl_mixed {
     int F[kCodes] = {}, variant[kCodes] = {}, grid[kCodes] = {}, occ[kCodes] = {};
     bool enabled[kCodes] = {};
     uint32_t *counts = nullptr, *cursors = nullptr, *buckets = nullptr, *idx = nullptr;
+    uint32_t *lv_counts = nullptr, *lv_cursors = nullptr, *lv_buckets = nullptr, *lv_idx = nullptr;
+    uint8_t* level = nullptr;
     unsigned long long* tiles = nullptr;
     size_t idx_cap = 0;
     unsigned long long* block_sum = nullptr;
@@ -228,10 +285,16 @@ void drop_graph(wl_mixed* m)
 int ensure_idx(wl_mixed* m, size_t n)
 {
     if (n <= m->idx_cap) return WL_OK;
-    if (m->idx) (void)hipFree(m->idx);
-    m->idx = nullptr;
+    void* ps[] = {m->idx, m->lv_idx, m->level};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    m->idx = m->lv_idx = nullptr;
+    m->level = nullptr;
     m->idx_cap = 0;
-    hipError_t e = hipMalloc((void**)&m->idx, std::max<size_t>(n, 1) * sizeof(uint32_t));
+    const size_t nn = std::max<size_t>(n, 1);
+    hipError_t e = hipMalloc((void**)&m->idx, nn * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->lv_idx, nn * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->level, nn);
     if (e != hipSuccess) return hipf(e, "hipMalloc");
     m->idx_cap = n;
     drop_graph(m);
@@ -277,6 +340,9 @@ int wl_mixed_create(int max_iter, unsigned flags, wl_mixed** out)
     e = hipMalloc((void**)&m->counts, (kCodes + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&m->cursors, (kCodes + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&m->buckets, 2 * kCodes * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->lv_counts, (kLevels + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->lv_cursors, (kLevels + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->lv_buckets, 2 * kLevels * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&m->tiles, kCodes * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void**)&m->descs, kCodes * sizeof(EncodeDesc));
     if (e == hipSuccess) e = hipMemcpy(m->descs, descs.data(), kCodes * sizeof(EncodeDesc), hipMemcpyHostToDevice);
@@ -298,7 +364,8 @@ int wl_mixed_destroy(wl_mixed* m)
 {
     if (!m) return WL_OK;
     drop_graph(m);
-    void* ps[] = {m->counts, m->cursors, m->buckets, m->idx, m->tiles, m->block_sum, m->descs};
+    void* ps[] = {m->counts, m->cursors, m->buckets, m->idx, m->tiles, m->block_sum, m->descs,
+                  m->lv_counts, m->lv_cursors, m->lv_buckets, m->lv_idx, m->level};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     for (int r = 0; r < kNumRates; r++)
@@ -385,11 +452,26 @@ static int capture_decode(wl_mixed* m, int learned, int fallback_iters, const ui
     if (e != hipSuccess) return hipf(e, "begin capture");
     const unsigned long long n = nframes;
     const unsigned blocks = (unsigned)std::min<unsigned long long>((n + 255) / 256, (unsigned long long)m->num_cu * 8);
-    hipLaunchKernelGGL(mixed_prep_kernel, dim3(1), dim3(128), 0, s, m->counts, m->cursors, m->tiles);
-    hipLaunchKernelGGL(mixed_hist_kernel, dim3(blocks), dim3(256), 0, s, cfg, n, m->counts);
-    hipLaunchKernelGGL(mixed_scan_kernel, dim3(1), dim3(64), 0, s, m->counts, m->buckets);
     const unsigned sblocks = (unsigned)((n + 256 * kScatterPer - 1) / (256 * kScatterPer));
-    hipLaunchKernelGGL(mixed_scatter_kernel, dim3(sblocks), dim3(256), 0, s, cfg, n, m->buckets, m->cursors, m->idx);
+    hipLaunchKernelGGL(mixed_prep_kernel, dim3(1), dim3(256), 0, s, m->counts, m->cursors, m->lv_counts,
+                       m->lv_cursors, m->tiles);
+    // Reliability order: only where frames stop early (with fixed iterations
+    // every frame costs the same).
+    const bool ordered = (learned || (m->flags & WL_FLAG_EARLY_STOP)) && !(m->flags & WL_FLAG_MIXED_UNORDERED);
+    if (ordered) {
+        const unsigned lblocks = (unsigned)std::min<unsigned long long>((n + 3) / 4, (unsigned long long)m->num_cu * 16);
+        hipLaunchKernelGGL(mixed_level_kernel, dim3(lblocks), dim3(256), 0, s, cfg,
+                           reinterpret_cast<const unsigned long long*>(frame_off), llr, n, m->level);
+        hipLaunchKernelGGL(bucket_hist_kernel<kLevels>, dim3(blocks), dim3(256), 0, s, m->level, n, m->lv_counts);
+        hipLaunchKernelGGL(bucket_scan_kernel<kLevels>, dim3(1), dim3(64), 0, s, m->lv_counts, m->lv_buckets);
+        hipLaunchKernelGGL(bucket_scatter_kernel<kLevels>, dim3(sblocks), dim3(256), 0, s, m->level,
+                           (const uint32_t*)nullptr, n, m->lv_buckets, m->lv_cursors, m->lv_idx);
+    }
+    hipLaunchKernelGGL(bucket_hist_kernel<kCodes>, dim3(blocks), dim3(256), 0, s, cfg, n, m->counts);
+    hipLaunchKernelGGL(bucket_scan_kernel<kCodes>, dim3(1), dim3(64), 0, s, m->counts, m->buckets);
+    hipLaunchKernelGGL(bucket_scatter_kernel<kCodes>, dim3(sblocks), dim3(256), 0, s, cfg,
+                       ordered ? (const uint32_t*)m->lv_idx : (const uint32_t*)nullptr, n, m->buckets, m->cursors,
+                       m->idx);
     e = hipEventRecord(m->ev_fork, s);
     for (int i = 0; i < kSideStreams && e == hipSuccess; i++) e = hipStreamWaitEvent(m->side[i], m->ev_fork, 0);
     // Longest first onto the least-loaded side stream, by edges per frame

@@ -10,8 +10,9 @@
 //     the shift folded into the instruction offset;
 //   * the check-to-variable messages of a lane's rows live in VGPRs
 //     (sum of layer degrees per lane, 76..88 floats);
-//   * min1 / min2 / sign parity are reduced in registers per lane: min2 via
-//     med3(min1, min2, |L|), and an edge receives min2 iff |L| == min1, which
+//   * min1 / min2 / sign parity are reduced in registers per lane: min1 / min2
+//     are the two smallest |L| of the row (min3 / med3 over slot triples,
+//     merged into the running pair), and an edge receives min2 iff |L| == min1, which
 //     equals the reference's first-strict-argmin choice (ties give min2 == min1);
 //   * the normalised message step runs in FP64 exactly as the reference's
 //     float*double expression (:220-223): one rounding to float of
@@ -359,13 +360,12 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     };
 
     // Failing parity checks among this lane's rows (lib/ldpc_decoder.cc:67-104
-    // over hd = rx >= 0).
-    // use_sel: address the rows like the layers do (one select per wrapped
-    // slot); used for the final check only -- inside the iteration loop that
-    // form raises the kernel's register count, so the early-stop check keeps
-    // the compare-and-select addressing
-    auto fail_rows = [&](auto selc) -> int {
-        constexpr bool use_sel = decltype(selc)::value;
+    // over hd = rx >= 0), addressed like the layers (one select per wrapped
+    // slot).  Each slot's compare is ballotted into a wave-wide mask and the
+    // masks are XORed in scalar registers in slot order; the barrier on the
+    // running mask keeps each read next to its compare, so the check adds no
+    // live registers beside the messages.
+    auto fail_rows = [&]() -> int {
         int fails = 0;
         static_for<0, NL>([&](auto Lc) {
             constexpr int L = decltype(Lc)::value;
@@ -373,13 +373,16 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             if constexpr ((VAR & VAR_NOHOIST) != 0) asm volatile("" : "+v"(rr));
             const uint32_t ob0 = lds_addr(smem) + (uint32_t)(kLdsPad + ((active ? f : 0) * N + r) * 4);
             const uint32_t obW = ob0 - (uint32_t)(Z * 4);
-            const uint64_t* mt = &WrapMasks<RATE, Z>::table.m[wphase][L][0];
-            if constexpr (use_sel && (VAR & VAR_SMASK) != 0) asm volatile("" : "+s"(mt));
-            uint32_t p = 0;
+            // the mask table through an opaque constant-space pointer: scalar
+            // loads next to their use rather than hoisted out of the loop
+            typedef const __attribute__((address_space(4))) uint64_t cmask;
+            cmask* mt = (cmask*)&WrapMasks<RATE, Z>::table.m[wphase][L][0];
+            if constexpr ((VAR & VAR_SMASK) != 0) asm volatile("" : "+s"(mt));
+            uint64_t pm = 0;
             static_for<0, CD::deg(L)>([&](auto Jc) {
                 constexpr int J = decltype(Jc)::value;
                 float xv;
-                if constexpr (use_sel && (VAR & VAR_SEL) != 0 && (VAR & VAR_SMASK) != 0) {
+                if constexpr ((VAR & VAR_SEL) != 0 && (VAR & VAR_SMASK) != 0) {
                     constexpr int b = CD::blk(L, J);
                     constexpr int s = CD::sh(L, J);
                     const uint32_t o = s == 0 ? ob0 : select_by_mask(ob0, obW, mt[J]);
@@ -389,9 +392,10 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
                 } else {
                     xv = *at(Lc, Jc);
                 }
-                p ^= (xv >= 0.0f) ? 1u : 0u;
+                pm ^= __builtin_amdgcn_ballot_w64(xv >= 0.0f);
+                if constexpr ((J & 3) == 3) asm volatile("" : "+s"(pm));
             });
-            fails += (int)p;
+            fails += __builtin_amdgcn_inverse_ballot_w64(pm) ? 1 : 0;
         });
         return fails;
     };
@@ -400,7 +404,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     int it_done = 0;
     if (a.early_stop) {
         // initial hard decision + syndrome (lib/ldpc_decoder.cc:130-139)
-        if (active && fail_rows(std::false_type{})) cnt[f] = 1;
+        if (active && fail_rows()) cnt[f] = 1;
         __syncthreads();
         if (active && cnt[f] == 0) done = true;
         __syncthreads();
@@ -414,7 +418,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         });
         if (a.early_stop) {
             // lib/ldpc_decoder.cc:242-251
-            if (!done && fail_rows(std::false_type{})) cnt[f] = 1;
+            if (!done && fail_rows()) cnt[f] = 1;
             __syncthreads();
             if (!done && cnt[f] == 0) {
                 done = true;
@@ -430,7 +434,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     __syncthreads();
     if (a.synd || a.ret) {
         if (active) {
-            const int fr = fail_rows(std::true_type{});
+            const int fr = fail_rows();
             if (fr) atomicAdd(&cnt[f], (uint32_t)fr);
         }
     }

@@ -1,7 +1,7 @@
 """Mixed-(rate, z) batches over wl_mixed_* of the C ABI."""
 import ctypes as C
 
-from .capi import check, lib
+from .capi import WL_FLAG_EARLY_STOP, WL_FLAG_MIXED_UNORDERED, check, lib
 from .codec import _ptr, _stream
 
 NUM_Z = 19
@@ -12,9 +12,12 @@ def code_id(rate, z):
 
 
 class MixedDecoder:
-    def __init__(self, max_iter=10, early_stop=True, codes=None):
+    def __init__(self, max_iter=10, early_stop=True, codes=None, ordered=True):
+        """ordered=False: frames of a code bucket are not arranged by
+        reliability (WL_FLAG_MIXED_UNORDERED); results are the same."""
         h = C.c_void_p()
-        check(lib().wl_mixed_create(int(max_iter), 1 if early_stop else 0, C.byref(h)))
+        flags = (WL_FLAG_EARLY_STOP if early_stop else 0) | (0 if ordered else WL_FLAG_MIXED_UNORDERED)
+        check(lib().wl_mixed_create(int(max_iter), flags, C.byref(h)))
         self._h = h
         if codes is not None:
             self.set_codes(codes)
