@@ -158,9 +158,9 @@ int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nfr
  * at element frame_off[g] of a ragged buffer; its posteriors go to the same
  * offsets of post, its packed hard decisions to byte frame_off[g] / 8 of hard,
  * and ret / iters / synd to index g.  A decode is one hipGraph launch: frames
- * are bucketed by code on the device and each code's bucket is decoded by a
- * persistent kernel with that code's autotuned launch shape; results equal a
- * per-code wl_decode_f32_dev bit for bit.  With early stop (create flags),
+ * are bucketed by code on the device and the buckets of all rates at one z
+ * are decoded by one persistent kernel (each rate with its own code shape);
+ * results equal a per-code wl_decode_f32_dev bit for bit.  With early stop (create flags),
  * frames inside a bucket are arranged by a reliability estimate (mean |LLR|
  * of the first 256 LLRs) so that the frames one workgroup decodes together
  * tend to stop together; WL_FLAG_MIXED_UNORDERED turns that off (same
@@ -172,6 +172,9 @@ int wl_mixed_create(int max_iter, unsigned flags, wl_mixed** out);
 int wl_mixed_destroy(wl_mixed* m);
 /* Restrict the graph to the listed code ids (ncodes = 0: all 114). */
 int wl_mixed_set_codes(wl_mixed* m, const int* codes, int ncodes);
+/* Launch shape the graph uses for code (rate, z): frames per workgroup, code
+ * shape variant and persistent grid of the kernel that decodes all rates of
+ * that z. */
 int wl_mixed_get_launch(const wl_mixed* m, int rate, int z, int* frames_per_wg, int* variant, int* grid);
 /* frame_off [nframes] = exclusive prefix sum of n over cfg [nframes] (device). */
 int wl_mixed_offsets_dev(wl_mixed* m, const uint8_t* cfg, size_t nframes, uint64_t* frame_off, void* stream);

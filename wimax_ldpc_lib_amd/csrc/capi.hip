@@ -16,6 +16,7 @@
 #include "errors.hpp"
 #include "generic.hpp"
 #include "qc_host.hpp"
+#include "tuned.hpp"
 
 #define WL_VERSION_STR "wimax_ldpc_amd 0.1 (gfx950)"
 
@@ -99,18 +100,9 @@ int default_frames_per_wg(int z, int fmax)
     return best;
 }
 
-// Autotuned launch shapes, written by wimax_ldpc_lib_amd/autotune.py from
-// measurements on MI355X: {rate, z, frames_per_wg, variant}.
-struct Tuned {
-    int rate, z, f, variant;
-};
-const Tuned kTuned[] = {
-#include "tuned_table.inc"
-    {-1, 0, 0, 0}};
-
 bool tuned_lookup(int rate, int z, int* f, int* variant)
 {
-    for (const Tuned& t : kTuned)
+    for (const wl::TunedShape& t : wl::kTunedShapes)
         if (t.rate == rate && t.z == z) {
             *f = t.f;
             *variant = t.variant;

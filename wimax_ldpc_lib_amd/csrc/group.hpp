@@ -1,0 +1,160 @@
+// group.hpp -- one persistent decode kernel per lifting size z for mixed
+// batches: the buckets of all six rates at that z are one work list.
+//
+// A mixed batch holds a few thousand frames per code; one persistent kernel
+// per code then spends much of its time in its last round of tiles, where a
+// tile of frames that do not converge runs all iterations while the rest of
+// the chip waits.  Grouping the six codes of one z into one kernel (same n,
+// so one LDS layout and one block size) gives each kernel six times the
+// tiles.  Each rate keeps its autotuned code shape; the frames per workgroup
+// are shared by the group (the value most of its rates were tuned to).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "qc_code.hpp"
+#include "tdmp.hpp"
+#include "tuned.hpp"
+
+namespace wl {
+
+struct GroupArgs {
+    DecodeArgs a;                      // buffers, frame list, ragged offsets, iterations, early stop
+    const uint32_t* buckets;           // [2 * 114] {first slot, count} per code, on the device
+    unsigned long long* tile_counter;  // this group's tile counter (zeroed before the launch)
+    unsigned enabled;                  // bit r: code (r, z) is decoded
+};
+
+// The group's frames per workgroup: the most common tuned value over its
+// rates, ties to the smaller block; each rate's variant as tuned.
+template <int Z>
+struct GroupShape {
+    static constexpr int pick_f()
+    {
+        int best = 0, best_n = 0;
+        for (int r = 0; r < kNumRates; r++) {
+            const int f = tuned_f(r, Z) > 0 ? tuned_f(r, Z) : 1;
+            int n = 0;
+            for (int q = 0; q < kNumRates; q++) n += (tuned_f(q, Z) > 0 ? tuned_f(q, Z) : 1) == f ? 1 : 0;
+            if (n > best_n || (n == best_n && f < best)) {
+                best = f;
+                best_n = n;
+            }
+        }
+        return best;
+    }
+    static constexpr int F = pick_f();
+    static constexpr int threads = ((F * Z + 63) / 64) * 64;
+    static constexpr size_t lds = kLdsPad + (size_t)F * 24 * Z * sizeof(float);
+    template <int R>
+    static constexpr int var()
+    {
+        return VAR_LEAN;
+    }
+};
+
+template <int Z>
+__global__ __launch_bounds__(kMaxThreads) void tdmp_group_kernel(GroupArgs g)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    using S = GroupShape<Z>;
+    constexpr int F = S::F;
+    constexpr int zi = index_of_z(Z);
+    unsigned long long* slot = reinterpret_cast<unsigned long long*>(smem + kTileSlot);
+    // Tiles of rate 0 first, then rate 1, ...: the counter only grows, so
+    // every workgroup reaches the exit once the last tile is taken.  Tile
+    // counts per rate come from the device bucket counts, re-read per tile
+    // (uniform over the workgroup) rather than held across the six tiles.
+    for (;;) {
+        if (threadIdx.x == 0) *slot = atomicAdd(g.tile_counter, 1ull);
+        __syncthreads();
+        unsigned long long t = *slot;
+        __syncthreads();
+        int rate = -1;
+        unsigned long long cnt = 0;
+        for (int r = 0; r < kNumRates; r++) {
+            const unsigned long long c = ((g.enabled >> r) & 1u) ? g.buckets[2 * (r * kNumZ + zi) + 1] : 0ull;
+            const unsigned long long nt = (c + F - 1) / F;
+            if (t < nt) {
+                rate = r;
+                cnt = c;
+                break;
+            }
+            t -= nt;
+        }
+        if (rate < 0) break;   // uniform: t and the counts are the same in every thread
+        const unsigned long long frame0 = t * F;
+        const unsigned long long left = cnt - frame0;
+        const int fe = left < (unsigned long long)F ? (int)left : F;
+        static_for<0, kNumRates>([&](auto Rc) {
+            constexpr int R = decltype(Rc)::value;
+            if (rate != R) return;
+            DecodeArgs a = g.a;
+            a.bucket = g.buckets + 2 * (R * kNumZ + zi);
+            a.frames_per_wg = F;
+            tdmp_tile<R, Z, MODE_NMS, S::template var<R>(), true>(a, smem, frame0, fe);
+        });
+        __syncthreads();
+    }
+}
+
+template <int Z>
+hipError_t launch_group(const GroupArgs& g, int grid, hipStream_t stream)
+{
+    hipLaunchKernelGGL(tdmp_group_kernel<Z>, dim3((unsigned)grid), dim3(GroupShape<Z>::threads), GroupShape<Z>::lds,
+                       stream, g);
+    return hipGetLastError();
+}
+
+// workgroups of the group kernel resident per CU (0 on error)
+template <int Z>
+int occupancy_group()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tdmp_group_kernel<Z>, GroupShape<Z>::threads,
+                                                     GroupShape<Z>::lds) != hipSuccess)
+        return 0;
+    return nb;
+}
+
+using GroupLaunch = hipError_t (*)(const GroupArgs&, int, hipStream_t);
+using GroupOccupancy = int (*)();
+
+struct GroupEntry {
+    GroupLaunch launch;
+    GroupOccupancy occ;
+    int frames_per_wg;
+};
+
+// defined in decode_g<k>.hip, one per lifting size
+#define WL_DECLARE_GROUP(Z) const GroupEntry* group_entry_z##Z();
+WL_DECLARE_GROUP(24) WL_DECLARE_GROUP(28) WL_DECLARE_GROUP(32) WL_DECLARE_GROUP(36) WL_DECLARE_GROUP(40)
+WL_DECLARE_GROUP(44) WL_DECLARE_GROUP(48) WL_DECLARE_GROUP(52) WL_DECLARE_GROUP(56) WL_DECLARE_GROUP(60)
+WL_DECLARE_GROUP(64) WL_DECLARE_GROUP(68) WL_DECLARE_GROUP(72) WL_DECLARE_GROUP(76) WL_DECLARE_GROUP(80)
+WL_DECLARE_GROUP(84) WL_DECLARE_GROUP(88) WL_DECLARE_GROUP(92) WL_DECLARE_GROUP(96)
+#undef WL_DECLARE_GROUP
+
+inline const GroupEntry* group_entry(int z)
+{
+    switch (z) {
+#define WL_GROUP_CASE(Z) \
+    case Z: return group_entry_z##Z();
+        WL_GROUP_CASE(24) WL_GROUP_CASE(28) WL_GROUP_CASE(32) WL_GROUP_CASE(36) WL_GROUP_CASE(40)
+        WL_GROUP_CASE(44) WL_GROUP_CASE(48) WL_GROUP_CASE(52) WL_GROUP_CASE(56) WL_GROUP_CASE(60)
+        WL_GROUP_CASE(64) WL_GROUP_CASE(68) WL_GROUP_CASE(72) WL_GROUP_CASE(76) WL_GROUP_CASE(80)
+        WL_GROUP_CASE(84) WL_GROUP_CASE(88) WL_GROUP_CASE(92) WL_GROUP_CASE(96)
+#undef WL_GROUP_CASE
+    default: return nullptr;
+    }
+}
+
+}  // namespace wl
+
+#define WL_DEFINE_GROUP(Z)                                                                  \
+    namespace wl {                                                                          \
+    const GroupEntry* group_entry_z##Z()                                                    \
+    {                                                                                       \
+        static const GroupEntry e{&launch_group<Z>, &occupancy_group<Z>, GroupShape<Z>::F}; \
+        return &e;                                                                          \
+    }                                                                                       \
+    }

@@ -14,10 +14,12 @@
 //             a sequence of short runs of frames of similar reliability, so the
 //             frames of one workgroup tile tend to stop after similar numbers
 //             of iterations (a workgroup runs until its slowest frame stops)
-//   decode    one persistent kernel per code (graph siblings on 4 streams),
-//             each with its autotuned frames per workgroup and code shape,
-//             taking tiles of its bucket from an atomic counter; an empty
-//             bucket exits at once
+//   decode    one persistent kernel per lifting size z (group.hpp; graph
+//             siblings on 4 streams) taking tiles of the buckets of all
+//             rates at that z from one atomic counter, each rate with its own
+//             code shape; an empty group exits at once.  (The learned graph
+//             runs a per-code learned + re-queue kernel for each code with
+//             offsets.)
 // Per-frame results land at the frame's own index/offset, so the output does
 // not depend on bucket order and equals a per-code decode bit for bit.
 #include <hip/hip_runtime.h>
@@ -33,6 +35,7 @@
 #include "dispatch.hpp"
 #include "encoder.hpp"
 #include "errors.hpp"
+#include "group.hpp"
 
 namespace wl {
 
@@ -53,7 +56,7 @@ __global__ void mixed_prep_kernel(uint32_t* counts, uint32_t* cursors, uint32_t*
         lv_counts[i] = 0;
         lv_cursors[i] = 0;
     }
-    for (int i = threadIdx.x; i < kCodes; i += blockDim.x) tiles[i] = 0;
+    for (int i = threadIdx.x; i < kCodes + kNumZ; i += blockDim.x) tiles[i] = 0;
 }
 
 // Reliability level of frame g: the mean |LLR| of its first kLevelSample LLRs
@@ -231,7 +234,7 @@ struct wl_mixed {
     unsigned flags = 0;
     int num_cu = 0;
     const DecodeEntry* entries[kCodes] = {};
-    int F[kCodes] = {}, variant[kCodes] = {}, grid[kCodes] = {}, occ[kCodes] = {};
+    int F[kCodes] = {};
     bool enabled[kCodes] = {};
     uint32_t *counts = nullptr, *cursors = nullptr, *buckets = nullptr, *idx = nullptr;
     uint32_t *lv_counts = nullptr, *lv_cursors = nullptr, *lv_buckets = nullptr, *lv_idx = nullptr;
@@ -254,6 +257,7 @@ struct wl_mixed {
     float* offsets[kNumRates] = {};
     int T[kNumRates] = {};
     int ncu = 0;
+    int gocc[kNumZ] = {};   // resident workgroups per CU of each z group kernel
 };
 
 namespace {
@@ -329,21 +333,19 @@ int wl_mixed_create(int max_iter, unsigned flags, wl_mixed** out)
             delete m;
             return WL_EINVAL;
         }
-        m->F[c] = f;
-        m->variant[c] = v;
-        const int occ = m->entries[c]->occ[v](f);
-        m->occ[c] = occ;
-        m->grid[c] = std::max(1, occ) * m->num_cu;
+        m->F[c] = f;   // the learned + re-queue kernels use the code's own tuned shape
+        (void)v;
         m->enabled[c] = true;
         descs[c] = make_encode_desc_for(rate, z);
     }
+    for (int zi = 0; zi < kNumZ; zi++) m->gocc[zi] = group_entry(z_of_index(zi))->occ();
     e = hipMalloc((void**)&m->counts, (kCodes + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&m->cursors, (kCodes + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&m->buckets, 2 * kCodes * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&m->lv_counts, (kLevels + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&m->lv_cursors, (kLevels + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&m->lv_buckets, 2 * kLevels * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc((void**)&m->tiles, kCodes * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void**)&m->tiles, (kCodes + kNumZ) * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void**)&m->descs, kCodes * sizeof(EncodeDesc));
     if (e == hipSuccess) e = hipMemcpy(m->descs, descs.data(), kCodes * sizeof(EncodeDesc), hipMemcpyHostToDevice);
     for (int i = 0; i < kSideStreams && e == hipSuccess; i++) {
@@ -398,10 +400,10 @@ int wl_mixed_get_launch(const wl_mixed* m, int rate, int z, int* frames_per_wg, 
 {
     if (!m || !frames_per_wg || !variant || !grid) return mfail(WL_EINVAL, "null argument");
     if (rate < 0 || rate >= kNumRates || !valid_z(z)) return mfail(WL_EINVAL, "invalid rate/z");
-    const int c = rate * kNumZ + index_of_z(z);
-    *frames_per_wg = m->F[c];
-    *variant = m->variant[c];
-    *grid = m->grid[c];
+    const int zi = index_of_z(z);
+    *frames_per_wg = group_entry(z)->frames_per_wg;
+    *variant = 1;
+    *grid = std::max(1, m->gocc[zi]) * m->num_cu;
     return WL_OK;
 }
 
@@ -474,45 +476,73 @@ static int capture_decode(wl_mixed* m, int learned, int fallback_iters, const ui
                        m->idx);
     e = hipEventRecord(m->ev_fork, s);
     for (int i = 0; i < kSideStreams && e == hipSuccess; i++) e = hipStreamWaitEvent(m->side[i], m->ev_fork, 0);
-    // Longest first onto the least-loaded side stream, by edges per frame
-    // (z x nonzero base blocks): the per-code frame counts are only known on
-    // the device, so the estimate assumes codes of the batch equally likely.
-    int order[kCodes];
-    int norder = 0;
-    for (int c = 0; c < kCodes; c++)
-        if (m->enabled[c]) order[norder++] = c;
+    // Jobs: the learned + re-queue kernel of each code that has offsets (in
+    // the learned graph), and one group kernel per z for every other enabled
+    // code of that z.  Longest first onto the least-loaded side stream, by
+    // edges per frame (z x nonzero base blocks) summed over the job's codes:
+    // per-code frame counts are only known on the device, so the estimate
+    // takes the codes of the batch as equally likely.
     auto work = [](int c) {
         const LayerTable lt = make_layer_table(c / kNumZ);
         int nnz = 0;
         for (int l = 0; l < lt.nl; l++) nnz += lt.deg[l];
         return (long)(24 + 4 * (c % kNumZ)) * nnz;
     };
-    std::stable_sort(order, order + norder, [&](int x, int y) { return work(x) > work(y); });
+    struct Job {
+        int code;        // >= 0: per-code learned kernel; -1: group of z index zi
+        int zi;
+        unsigned mask;   // group: rates decoded
+        long w;
+    };
+    Job jobs[kCodes + kNumZ];
+    int njobs = 0;
+    for (int zi = 0; zi < kNumZ; zi++) {
+        Job gj{-1, zi, 0u, 0};
+        for (int rate = 0; rate < kNumRates; rate++) {
+            const int c = rate * kNumZ + zi;
+            if (!m->enabled[c]) continue;
+            if (learned && m->offsets[rate] && m->entries[c]->learned_fb) {
+                jobs[njobs++] = Job{c, zi, 0u, work(c)};
+            } else {
+                gj.mask |= 1u << rate;
+                gj.w += work(c);
+            }
+        }
+        if (gj.mask) jobs[njobs++] = gj;
+    }
+    std::stable_sort(jobs, jobs + njobs, [](const Job& x, const Job& y) { return x.w > y.w; });
+    DecodeArgs base{};
+    base.llr = llr;
+    base.post = post;
+    base.hard = hard;
+    base.ret = ret;
+    base.iters = iters;
+    base.synd = synd;
+    base.frame_idx = m->idx;
+    base.frame_off = reinterpret_cast<const unsigned long long*>(frame_off);
+    base.nframes = n;
+    base.max_iter = m->max_iter;
+    base.early_stop = (m->flags & WL_FLAG_EARLY_STOP) ? 1 : 0;
     long load[kSideStreams] = {};
-    for (int oi = 0; oi < norder && e == hipSuccess; oi++) {
-        const int c = order[oi];
-        DecodeArgs a{};
-        a.llr = llr;
-        a.post = post;
-        a.hard = hard;
-        a.ret = ret;
-        a.iters = iters;
-        a.synd = synd;
-        a.frame_idx = m->idx;
-        a.bucket = m->buckets + 2 * c;
-        a.frame_off = reinterpret_cast<const unsigned long long*>(frame_off);
-        a.tile_counter = m->tiles + c;
-        a.persistent_grid = (int)std::min<unsigned long long>((unsigned long long)m->grid[c],
-                                                              (n + m->F[c] - 1) / m->F[c]);
-        a.nframes = n;
-        a.frames_per_wg = m->F[c];
-        a.max_iter = m->max_iter;
-        a.early_stop = (m->flags & WL_FLAG_EARLY_STOP) ? 1 : 0;
-        const int rate = c / kNumZ;
+    for (int ji = 0; ji < njobs && e == hipSuccess; ji++) {
+        const Job& j = jobs[ji];
         const int si = (int)(std::min_element(load, load + kSideStreams) - load);
-        load[si] += work(c);
+        load[si] += j.w;
         hipStream_t ss = m->side[si];
-        if (learned && m->offsets[rate] && m->entries[c]->learned_fb) {
+        if (j.code < 0) {
+            GroupArgs g{};
+            g.a = base;
+            g.buckets = m->buckets;
+            g.tile_counter = m->tiles + kCodes + j.zi;
+            g.enabled = j.mask;
+            const int grid = std::max(1, m->gocc[j.zi]) * m->num_cu;
+            e = group_entry(z_of_index(j.zi))->launch(g, grid, ss);
+        } else {
+            const int c = j.code, rate = c / kNumZ;
+            DecodeArgs a = base;
+            a.bucket = m->buckets + 2 * c;
+            a.tile_counter = m->tiles + c;
+            a.frames_per_wg = m->F[c];
             a.offsets = m->offsets[rate];
             a.n_offset_iters = m->T[rate];
             a.fallback_iters = fallback_iters;
@@ -521,8 +551,6 @@ static int capture_decode(wl_mixed* m, int learned, int fallback_iters, const ui
             a.persistent_grid = (int)std::min<unsigned long long>((unsigned long long)std::max(1, occ) * m->num_cu,
                                                                   (n + m->F[c] - 1) / m->F[c]);
             e = m->entries[c]->learned_fb(a, ss);
-        } else {
-            e = m->entries[c]->nms[m->variant[c]](a, ss);
         }
     }
     for (int i = 0; i < kSideStreams && e == hipSuccess; i++) {

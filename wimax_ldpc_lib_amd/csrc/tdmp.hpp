@@ -97,6 +97,11 @@ struct WrapMasks {
     static constexpr Table table = make();
 };
 
+// the mask table seen through a constant-address-space pointer: a pointer
+// made opaque with an empty asm keeps the masks as scalar loads at their
+// point of use instead of constants hoisted out of the loops
+typedef const __attribute__((address_space(4))) uint64_t cmask_t;
+
 // this lane's bit of a wave-uniform mask set ? b : a (one v_cndmask on the
 // mask's scalar registers)
 __device__ __forceinline__ uint32_t select_by_mask(uint32_t a, uint32_t b, uint64_t mask)
@@ -255,6 +260,11 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         uint32_t ob0 = lds_addr(smem) + (uint32_t)(kLdsPad + ((active ? f : 0) * N + r) * 4);
         uint32_t obW = ob0 - (uint32_t)(Z * 4);
         if constexpr ((VAR & VAR_SEL) != 0) asm volatile("" : "+v"(ob0), "+v"(obW));
+        // with one wave phase (z = 32, 64) the masks are constants; opaque,
+        // they stay per-layer scalar loads rather than registers held (and,
+        // in the larger kernels, spilled) across the iteration loop
+        cmask_t* mtl = (cmask_t*)&WrapMasks<RATE, Z>::table.m[wphase][L][0];
+        if constexpr ((VAR & VAR_SMASK) != 0 && WrapMasks<RATE, Z>::P == 1) asm volatile("" : "+s"(mtl));
         auto atl = [&](auto Lc2, auto Jc2) {
             if constexpr ((VAR & VAR_SEL) != 0) {
                 constexpr int b = CD::blk(L, decltype(Jc2)::value);
@@ -263,7 +273,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
                 if constexpr (s == 0) {
                     o = ob0;
                 } else if constexpr ((VAR & VAR_SMASK) != 0) {
-                    o = select_by_mask(ob0, obW, WrapMasks<RATE, Z>::table.m[wphase][L][decltype(Jc2)::value]);
+                    o = select_by_mask(ob0, obW, mtl[decltype(Jc2)::value]);
                 } else {
                     o = rr >= Z - s ? obW : ob0;
                 }
@@ -373,10 +383,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             if constexpr ((VAR & VAR_NOHOIST) != 0) asm volatile("" : "+v"(rr));
             const uint32_t ob0 = lds_addr(smem) + (uint32_t)(kLdsPad + ((active ? f : 0) * N + r) * 4);
             const uint32_t obW = ob0 - (uint32_t)(Z * 4);
-            // the mask table through an opaque constant-space pointer: scalar
-            // loads next to their use rather than hoisted out of the loop
-            typedef const __attribute__((address_space(4))) uint64_t cmask;
-            cmask* mt = (cmask*)&WrapMasks<RATE, Z>::table.m[wphase][L][0];
+            cmask_t* mt = (cmask_t*)&WrapMasks<RATE, Z>::table.m[wphase][L][0];
             if constexpr ((VAR & VAR_SMASK) != 0) asm volatile("" : "+s"(mt));
             uint64_t pm = 0;
             static_for<0, CD::deg(L)>([&](auto Jc) {
