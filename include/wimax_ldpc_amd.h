@@ -92,8 +92,11 @@ int wl_decoder_destroy(wl_decoder* dec);
  * is one frame per CPU thread).  frames_per_wg: frames decoded together by
  * one workgroup (0 = the autotuned default); variant: code shape of the
  * normalised min-sum kernel (0: slot addresses held in registers, 1: slot
- * addresses recomputed per layer, fewer registers).  Results are identical
- * for every shape; create() starts from the autotuned table. */
+ * addresses recomputed per layer, fewer registers, 2: as 1 with each wave
+ * holding 32 rows of two frames -- only for z % 32 == 0 and
+ * frames_per_wg 2, set first; setting another frames_per_wg then switches
+ * to 1).  Results are identical for every shape; create() starts from the
+ * autotuned table. */
 int wl_decoder_set_frames_per_wg(wl_decoder* dec, int frames_per_wg);
 int wl_decoder_get_frames_per_wg(const wl_decoder* dec, int* out);
 int wl_decoder_set_variant(wl_decoder* dec, int variant);

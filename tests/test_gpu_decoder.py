@@ -196,3 +196,32 @@ def test_large_batch_properties(gpu, oracle_mod):
     hd = np.unpackbits(hard_np[:256], axis=1, bitorder="little")[:, :code.n]
     w = ((hd.astype(np.int64) @ H.T.astype(np.int64)) % 2).sum(axis=1)
     assert np.array_equal(w, synd.cpu().numpy()[:256])
+
+
+@pytest.mark.parametrize("rate,z", [(5, 96), (0, 64), (3, 32), (4, 96)])
+def test_two_halves_code_shape(gpu, oracle_mod, rate, z):
+    """Code shape 2 (each wave holds 32 rows of two frames, per-wave compile-time
+    slot addressing) equals the oracle in both stop modes, with a partial last
+    workgroup (odd frame count); it needs two frames per workgroup, and another
+    frames-per-workgroup value falls back to shape 1."""
+    import wimax_ldpc_lib_amd as W
+    code = oracle_mod.Code(rate, z)
+    if rate == 4:
+        llr = np.random.default_rng(z).normal(0.8, 2.0, (33, code.n)).astype(np.float32)
+    else:
+        llr, _ = noisy_llr(code, 33, {0: 1.8, 3: 2.8, 5: 3.4}[rate], seed=500 + z)
+    for early in (True, False):
+        o = code.decode(llr, max_iter=9, early_stop=early)
+        dec = W.ldpc_decoder(rate, z, 9, early_stop=early)
+        dec.variant = 0
+        dec.frames_per_wg = 3
+        with pytest.raises(W.WLError):
+            dec.variant = 2
+        dec.frames_per_wg = 2
+        dec.variant = 2
+        assert dec.variant == 2
+        _compare(_gpu_decode(gpu, dec, llr), o, "shape 2 rate %d z %d early %d" % (rate, z, early))
+        dec.frames_per_wg = 1
+        assert dec.variant == 1
+    with pytest.raises(W.WLError):
+        W.ldpc_decoder(0, 24, 5).variant = 2      # z % 32 != 0: no such shape

@@ -3,14 +3,15 @@
 For every code it times the normalised min-sum kernel (exactly `--iters`
 iterations, no early stop, so the work is the same for every shape) over
 each code-shape variant and every frames-per-workgroup value the kernel
-accepts, on a batch of seeded channel frames holding about 150 Mbit of
+accepts (shape 2, where z % 32 == 0, at two frames per workgroup only),
+on a batch of seeded channel frames holding about 150 Mbit of
 codeword bits, and keeps the fastest.  Every shape is checked to give the
 same hard decisions as the first one.  Results go to
 
   wimax_ldpc_lib_amd/csrc/tuned_table.inc   compiled into the library
   wimax_ldpc_lib_amd/tuned.json             the measured times
 
-    python -m wimax_ldpc_lib_amd.autotune [--rates 0,1,...] [--zs 24,...]
+    python -m wimax_ldpc_lib_amd.autotune [--rates 0,1,...] [--zs 24,...] [--merge tuned.json]
 """
 import argparse
 import json
@@ -33,6 +34,7 @@ def tune_one(torch, W, rate, z, iters, reps, bits_target):
     ret = torch.empty(F, dtype=torch.uint8, device="cuda")
     ref = None
     results = []
+    dec.variant = 0   # any frames-per-workgroup value is allowed with shapes 0 and 1
     fmax = 1
     while True:
         try:
@@ -40,26 +42,29 @@ def tune_one(torch, W, rate, z, iters, reps, bits_target):
             fmax += 1
         except W.WLError:
             break
-    for variant in (0, 1):
+    shapes = [(v, f) for v in (0, 1) for f in range(1, fmax + 1)]
+    if z % 32 == 0 and fmax >= 2:
+        shapes.append((2, 2))   # two-halves lane mapping: two frames per workgroup only
+    for variant, f in shapes:
+        dec.variant = 0
+        dec.frames_per_wg = f
         dec.variant = variant
-        for f in range(1, fmax + 1):
-            dec.frames_per_wg = f
+        dec.decode_dev(llr, None, hard, ret)
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
             dec.decode_dev(llr, None, hard, ret)
-            torch.cuda.synchronize()
-            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            s.record()
-            for _ in range(reps):
-                dec.decode_dev(llr, None, hard, ret)
-            e.record()
-            torch.cuda.synchronize()
-            ms = s.elapsed_time(e) / reps
-            h = hard.cpu()
-            if ref is None:
-                ref = h
-            elif not torch.equal(ref, h):
-                raise RuntimeError("rate %d z %d variant %d F %d: hard decisions differ" % (rate, z, variant, f))
-            results.append({"variant": variant, "f": f, "ms": ms,
-                            "info_gbps": F * dec.k / (ms * 1e-3) / 1e9})
+        e.record()
+        torch.cuda.synchronize()
+        ms = s.elapsed_time(e) / reps
+        h = hard.cpu()
+        if ref is None:
+            ref = h
+        elif not torch.equal(ref, h):
+            raise RuntimeError("rate %d z %d variant %d F %d: hard decisions differ" % (rate, z, variant, f))
+        results.append({"variant": variant, "f": f, "ms": ms,
+                        "info_gbps": F * dec.k / (ms * 1e-3) / 1e9})
     best = min(results, key=lambda r: r["ms"])
     return {"rate": rate, "z": z, "frames": F, "iters": iters, "best": best, "all": results}
 
@@ -74,6 +79,9 @@ def main():
     ap.add_argument("--out-dir", default=PKG)
     ap.add_argument("--from-json", default=None,
                     help="skip measuring: write the table from a tuned.json of an earlier run")
+    ap.add_argument("--merge", default=None,
+                    help="tuned.json of an earlier run: codes measured now replace their entries, "
+                         "the others are kept")
     a = ap.parse_args()
     if a.from_json:
         write_outputs(json.load(open(a.from_json)), a.out_dir, a.iters)
@@ -92,6 +100,11 @@ def main():
             b = r["best"]
             print("rate %d z %2d: variant %d F %2d  %.3f ms  %.2f info Gbit/s  (%.0f s)"
                   % (rate, z, b["variant"], b["f"], b["ms"], b["info_gbps"], time.time() - t0), flush=True)
+    if a.merge:
+        new = {(r["rate"], r["z"]): r for r in out}
+        old = json.load(open(a.merge))
+        out = [new.pop((r["rate"], r["z"]), r) for r in old] + list(new.values())
+        out.sort(key=lambda r: (r["rate"], r["z"]))
     write_outputs(out, a.out_dir, a.iters)
 
 

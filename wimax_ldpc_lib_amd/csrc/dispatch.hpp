@@ -11,7 +11,11 @@ namespace wl {
 using DecodeLaunch = hipError_t (*)(const DecodeArgs&, hipStream_t);
 using DecodeOccupancy = int (*)(int frames_per_wg);
 
-constexpr int kNumVariants = 2;   // code-shape variants of the NMS kernel (tdmp.hpp VAR_*)
+// code-shape variants of the NMS kernel (tdmp.hpp VAR_*): 0 hoisted row
+// addresses, 1 VAR_LEAN, 2 VAR_LEAN_W2 (two frames per workgroup, z % 32 == 0
+// only; null elsewhere)
+constexpr int kNumVariants = 3;
+constexpr int kVariantW2 = 2;
 
 struct DecodeEntry {
     DecodeLaunch nms[kNumVariants];  // normalised min-sum, the reference's arithmetic
@@ -28,6 +32,20 @@ template <int RATE, int Z>
 constexpr DecodeLaunch fb_launcher()
 {
     if constexpr (Z == 24 || Z == 96) return &launch_tdmp<RATE, Z, MODE_LEARNED_FB, VAR_LEAN>;
+    else return nullptr;
+}
+
+template <int RATE, int Z>
+constexpr DecodeLaunch w2_launcher()
+{
+    if constexpr (Z % 32 == 0) return &launch_tdmp<RATE, Z, MODE_NMS, VAR_LEAN_W2>;
+    else return nullptr;
+}
+
+template <int RATE, int Z>
+constexpr DecodeOccupancy w2_occupancy()
+{
+    if constexpr (Z % 32 == 0) return &occupancy_tdmp<RATE, Z, MODE_NMS, VAR_LEAN_W2, true>;
     else return nullptr;
 }
 
@@ -66,11 +84,13 @@ constexpr auto make_decode_table(std::integer_sequence<int, ZI...>)
 {
     struct Arr { DecodeEntry e[sizeof...(ZI)]; };
     return Arr{{DecodeEntry{{&launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0>,
-                             &launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN>},
+                             &launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN>,
+                             w2_launcher<RATE, z_of_index(ZI)>()},
                             &launch_tdmp<RATE, z_of_index(ZI), MODE_LEARNED, 0>,
                             max_frames_per_wg<RATE, z_of_index(ZI)>(),
                             {&occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0, true>,
-                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN, true>},
+                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN, true>,
+                             w2_occupancy<RATE, z_of_index(ZI)>()},
                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_LEARNED, 0, true>,
                             fb_launcher<RATE, z_of_index(ZI)>(),
                             fb_occupancy<RATE, z_of_index(ZI)>()}...}};

@@ -6,8 +6,8 @@
 // tile of frames that do not converge runs all iterations while the rest of
 // the chip waits.  Grouping the six codes of one z into one kernel (same n,
 // so one LDS layout and one block size) gives each kernel six times the
-// tiles.  Each rate keeps its autotuned code shape; the frames per workgroup
-// are shared by the group (the value most of its rates were tuned to).
+// tiles.  The frames per workgroup are shared by the group (the value most of
+// its rates were tuned to).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,30 +26,42 @@ struct GroupArgs {
 };
 
 // The group's frames per workgroup: the most common tuned value over its
-// rates, ties to the smaller block; each rate's variant as tuned.
+// rates, ties to the smaller block.
 template <int Z>
 struct GroupShape {
+    // a rate's tuned F counts unless it was tuned to the two-halves shape,
+    // which fixes F = 2 (if all were, F = 2)
+    static constexpr int rate_f(int r)
+    {
+        return tuned_variant(r, Z) == 2 ? 0 : (tuned_f(r, Z) > 0 ? tuned_f(r, Z) : 1);
+    }
     static constexpr int pick_f()
     {
         int best = 0, best_n = 0;
         for (int r = 0; r < kNumRates; r++) {
-            const int f = tuned_f(r, Z) > 0 ? tuned_f(r, Z) : 1;
+            const int f = rate_f(r);
+            if (f == 0) continue;
             int n = 0;
-            for (int q = 0; q < kNumRates; q++) n += (tuned_f(q, Z) > 0 ? tuned_f(q, Z) : 1) == f ? 1 : 0;
+            for (int q = 0; q < kNumRates; q++) n += rate_f(q) == f ? 1 : 0;
             if (n > best_n || (n == best_n && f < best)) {
                 best = f;
                 best_n = n;
             }
         }
-        return best;
+        return best > 0 ? best : 2;
     }
     static constexpr int F = pick_f();
+    // the register-lean shape for every rate: the hoisted-address shape
+    // pushes the combined kernel past 168 VGPRs, and the two-halves shape
+    // (six rates x one loop copy per wave) spills scalar registers and
+    // measured slower here
+    static constexpr int VAR = VAR_LEAN;
     static constexpr int threads = ((F * Z + 63) / 64) * 64;
-    static constexpr size_t lds = kLdsPad + (size_t)F * 24 * Z * sizeof(float);
+    static constexpr size_t lds = kLdsPad + (size_t)F * lds_frame_stride<24 * Z, VAR>() * sizeof(float);
     template <int R>
     static constexpr int var()
     {
-        return VAR_LEAN;
+        return VAR;
     }
 };
 

@@ -134,9 +134,24 @@ enum { MODE_NMS = 0, MODE_LEARNED = 1, MODE_LEARNED_FB = 2 };
 //   VAR_SMASK    with VAR_SEL: the per-slot "row wraps" lane masks come from a
 //                compile-time table (scalar loads) instead of a compare per
 //                slot, so the select is the only VALU op of the address
-enum { VAR_NOHOIST = 1, VAR_REREAD = 2, VAR_SEL = 4, VAR_SMASK = 8 };
+//   VAR_W2       with VAR_SEL, two frames per workgroup and z % 32 == 0: lane
+//                l of wave w owns row 32 w + l % 32 of frame l / 32, and each
+//                wave runs its own compile-time copy of the iteration loop.
+//                A slot's wrap boundary z - shift then falls inside at most
+//                one wave's 32 rows, so the other waves address it without a
+//                select.  Frames sit 32 floats apart beyond n in LDS so the
+//                two halves of a wave read different banks.
+enum { VAR_NOHOIST = 1, VAR_REREAD = 2, VAR_SEL = 4, VAR_SMASK = 8, VAR_W2 = 16 };
 // the low-register code shape of the tuned variants
 constexpr int VAR_LEAN = VAR_NOHOIST | VAR_SEL | VAR_SMASK;
+constexpr int VAR_LEAN_W2 = VAR_LEAN | VAR_W2;
+
+// floats between consecutive frames in LDS
+template <int N, int VAR>
+constexpr int lds_frame_stride()
+{
+    return (VAR & VAR_W2) != 0 ? N + 32 : N;
+}
 
 template <int RATE, int Z>
 constexpr int max_frames_per_wg()
@@ -161,14 +176,23 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     constexpr int NL = CD::NL;
     constexpr int N = CD::N;
     constexpr int C = CD::C;
+    constexpr bool W2 = (VAR & VAR_W2) != 0;
+    static_assert(!W2 || (Z % 32 == 0 && (VAR & VAR_SEL) != 0), "VAR_W2 needs z % 32 == 0 and VAR_SEL");
+    constexpr int NS = lds_frame_stride<N, VAR>();
 
     uint32_t* cnt = reinterpret_cast<uint32_t*>(smem);
     float* rxall = reinterpret_cast<float*>(smem + kLdsPad);
 
     const int tid = threadIdx.x;
     const int F = a.frames_per_wg;
-    const int f = tid / Z;
-    const int r = tid - f * Z;
+    int f, r;
+    if constexpr (W2) {
+        f = (tid >> 5) & 1;
+        r = ((tid >> 6) << 5) + (tid & 31);
+    } else {
+        f = tid / Z;
+        r = tid - f * Z;
+    }
     const bool active = f < Feff;
     // wave phase of the wrap-mask table (uniform over the wave)
     const int wphase = (__builtin_amdgcn_readfirstlane(tid) >> 6) % WrapMasks<RATE, Z>::P;
@@ -203,13 +227,22 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             for (int u = 0; u < 8; u++)
                 if (i0 + u * bd < total4) v[u] = __builtin_nontemporal_load(src + i0 + u * bd);
 #pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (i0 + u * bd < total4) dst[i0 + u * bd] = v[u];
+            for (int u = 0; u < 8; u++) {
+                const int i = i0 + u * bd;
+                if (i < total4) {
+                    if constexpr (NS == N) {
+                        dst[i] = v[u];
+                    } else {
+                        const int s = i / (N / 4);
+                        dst[i + s * ((NS - N) / 4)] = v[u];
+                    }
+                }
+            }
         }
     } else {
         for (int s = 0; s < Feff; s++) {
             const f32x4* src = reinterpret_cast<const f32x4*>(a.llr + goff(s));
-            f32x4* dst = reinterpret_cast<f32x4*>(rxall + s * N);
+            f32x4* dst = reinterpret_cast<f32x4*>(rxall + s * NS);
             for (int i = tid; i < N / 4; i += blockDim.x) dst[i] = src[i];
         }
     }
@@ -217,7 +250,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
 
     // Row base pointers: slot (l, j) of this lane's row reads
     // rx[blk*Z + r + s] or, once r + s wraps, rx[blk*Z + r + s - Z].
-    float* const frame_base = rxall + (active ? f : 0) * N;
+    float* const frame_base = rxall + (active ? f : 0) * NS;
     float* const base0 = frame_base + r;
     float* const baseW = base0 - Z;
 
@@ -250,14 +283,46 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         static_for<0, C>([&](auto Jc) { msg[decltype(Lc)::value][decltype(Jc)::value] = 0.0f; });
     });
 
+    // LDS byte address of slot (L, J) of this lane's row: one of the two row
+    // bases (ob0: rows r, obW: r - Z) plus the slot's constant offset.  With
+    // VAR_W2, W is this wave's index (rows 32 W .. 32 W + 31): where all of
+    // them wrap or none does, the base is chosen at compile time; otherwise
+    // (and without VAR_W2) one select on the lane mask.
+    auto sel_addr = [&](auto Lc2, auto Jc2, auto Wc, uint32_t ob0, uint32_t obW, cmask_t* mt) -> uint32_t {
+        constexpr int L2 = decltype(Lc2)::value;
+        constexpr int J2 = decltype(Jc2)::value;
+        constexpr int b = CD::blk(L2, J2);
+        constexpr int s = CD::sh(L2, J2);
+        constexpr uint32_t off = (uint32_t)((b * Z + s) * 4);
+        if constexpr (s == 0) {
+            return ob0 + off;
+        } else if constexpr (W2) {
+            constexpr int lo = 32 * decltype(Wc)::value, bnd = Z - s;   // rows >= bnd wrap
+            if constexpr (bnd >= lo + 32) {
+                return ob0 + off;
+            } else if constexpr (bnd <= lo) {
+                return obW + off;
+            } else {
+                constexpr uint32_t m32 = ~0u << (bnd - lo);
+                uint64_t m = (uint64_t)m32 | ((uint64_t)m32 << 32);
+                // in the extended kernels (more scalar state) materialised
+                // here, two scalar moves, rather than hoisted and spilled
+                if constexpr (EXT) asm volatile("" : "+s"(m));
+                return select_by_mask(ob0, obW, m) + off;
+            }
+        } else {
+            return select_by_mask(ob0, obW, mt[J2]) + off;
+        }
+    };
+
     // One layer for this lane's row (lib/ldpc_decoder.cc:143-240).
-    auto layer = [&](auto Lc, int t) {
+    auto layer = [&](auto Lc, int t, auto Wc) {
         constexpr int L = decltype(Lc)::value;
         constexpr int D = CD::deg(L);
         int rr = r;
         if constexpr ((VAR & VAR_NOHOIST) != 0) asm volatile("" : "+v"(rr));
         // VAR_SEL: LDS byte addresses of the two row bases, opaque per layer
-        uint32_t ob0 = lds_addr(smem) + (uint32_t)(kLdsPad + ((active ? f : 0) * N + r) * 4);
+        uint32_t ob0 = lds_addr(smem) + (uint32_t)(kLdsPad + ((active ? f : 0) * NS + r) * 4);
         uint32_t obW = ob0 - (uint32_t)(Z * 4);
         if constexpr ((VAR & VAR_SEL) != 0) asm volatile("" : "+v"(ob0), "+v"(obW));
         // with one wave phase (z = 32, 64) the masks are constants; opaque,
@@ -266,17 +331,12 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         cmask_t* mtl = (cmask_t*)&WrapMasks<RATE, Z>::table.m[wphase][L][0];
         if constexpr ((VAR & VAR_SMASK) != 0 && WrapMasks<RATE, Z>::P == 1) asm volatile("" : "+s"(mtl));
         auto atl = [&](auto Lc2, auto Jc2) {
-            if constexpr ((VAR & VAR_SEL) != 0) {
+            if constexpr ((VAR & VAR_SEL) != 0 && ((VAR & VAR_SMASK) != 0 || W2)) {
+                return lds_ptr(sel_addr(Lc2, Jc2, Wc, ob0, obW, mtl));
+            } else if constexpr ((VAR & VAR_SEL) != 0) {
                 constexpr int b = CD::blk(L, decltype(Jc2)::value);
                 constexpr int s = CD::sh(L, decltype(Jc2)::value);
-                uint32_t o;
-                if constexpr (s == 0) {
-                    o = ob0;
-                } else if constexpr ((VAR & VAR_SMASK) != 0) {
-                    o = select_by_mask(ob0, obW, mtl[decltype(Jc2)::value]);
-                } else {
-                    o = rr >= Z - s ? obW : ob0;
-                }
+                const uint32_t o = s == 0 ? ob0 : (rr >= Z - s ? obW : ob0);
                 return lds_ptr(o + (uint32_t)((b * Z + s) * 4));
             } else if constexpr ((VAR & VAR_NOHOIST) != 0) {
                 return at_r(Lc2, Jc2, rr);
@@ -375,13 +435,13 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     // masks are XORed in scalar registers in slot order; the barrier on the
     // running mask keeps each read next to its compare, so the check adds no
     // live registers beside the messages.
-    auto fail_rows = [&]() -> int {
+    auto fail_rows = [&](auto Wc) -> int {
         int fails = 0;
         static_for<0, NL>([&](auto Lc) {
             constexpr int L = decltype(Lc)::value;
             int rr = r;
             if constexpr ((VAR & VAR_NOHOIST) != 0) asm volatile("" : "+v"(rr));
-            const uint32_t ob0 = lds_addr(smem) + (uint32_t)(kLdsPad + ((active ? f : 0) * N + r) * 4);
+            const uint32_t ob0 = lds_addr(smem) + (uint32_t)(kLdsPad + ((active ? f : 0) * NS + r) * 4);
             const uint32_t obW = ob0 - (uint32_t)(Z * 4);
             cmask_t* mt = (cmask_t*)&WrapMasks<RATE, Z>::table.m[wphase][L][0];
             if constexpr ((VAR & VAR_SMASK) != 0) asm volatile("" : "+s"(mt));
@@ -389,11 +449,8 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             static_for<0, CD::deg(L)>([&](auto Jc) {
                 constexpr int J = decltype(Jc)::value;
                 float xv;
-                if constexpr ((VAR & VAR_SEL) != 0 && (VAR & VAR_SMASK) != 0) {
-                    constexpr int b = CD::blk(L, J);
-                    constexpr int s = CD::sh(L, J);
-                    const uint32_t o = s == 0 ? ob0 : select_by_mask(ob0, obW, mt[J]);
-                    xv = *lds_ptr(o + (uint32_t)((b * Z + s) * 4));
+                if constexpr ((VAR & VAR_SEL) != 0 && ((VAR & VAR_SMASK) != 0 || W2)) {
+                    xv = *lds_ptr(sel_addr(Lc, Jc, Wc, ob0, obW, mt));
                 } else if constexpr ((VAR & VAR_NOHOIST) != 0) {
                     xv = *at_r(Lc, Jc, rr);
                 } else {
@@ -409,42 +466,55 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
 
     bool done = !active;
     int it_done = 0;
-    if (a.early_stop) {
-        // initial hard decision + syndrome (lib/ldpc_decoder.cc:130-139)
-        if (active && fail_rows()) cnt[f] = 1;
-        __syncthreads();
-        if (active && cnt[f] == 0) done = true;
-        __syncthreads();
-        if (tid < F) cnt[tid] = 0;
-    }
-    int any = __syncthreads_or(!done);
-    for (int it = 0; it < a.max_iter && any; it++) {
-        static_for<0, NL>([&](auto Lc) {
-            if (!done) layer(Lc, it);
-            __syncthreads();
-        });
+    // Initial check, iterations and final syndrome.  W: this wave's index
+    // with VAR_W2 (each wave runs its own copy; every copy passes the same
+    // barriers), -1 otherwise.
+    auto run = [&](auto Wc) {
         if (a.early_stop) {
-            // lib/ldpc_decoder.cc:242-251
-            if (!done && fail_rows()) cnt[f] = 1;
+            // initial hard decision + syndrome (lib/ldpc_decoder.cc:130-139)
+            if (active && fail_rows(Wc)) cnt[f] = 1;
             __syncthreads();
-            if (!done && cnt[f] == 0) {
-                done = true;
-                it_done = it + 1;
-            }
-            any = __syncthreads_or(!done);
+            if (active && cnt[f] == 0) done = true;
+            __syncthreads();
             if (tid < F) cnt[tid] = 0;
         }
+        int any = __syncthreads_or(!done);
+        for (int it = 0; it < a.max_iter && any; it++) {
+            static_for<0, NL>([&](auto Lc) {
+                if (!done) layer(Lc, it, Wc);
+                __syncthreads();
+            });
+            if (a.early_stop) {
+                // lib/ldpc_decoder.cc:242-251
+                if (!done && fail_rows(Wc)) cnt[f] = 1;
+                __syncthreads();
+                if (!done && cnt[f] == 0) {
+                    done = true;
+                    it_done = it + 1;
+                }
+                any = __syncthreads_or(!done);
+                if (tid < F) cnt[tid] = 0;
+            }
+        }
+        __syncthreads();
+        if (a.synd || a.ret) {
+            if (active) {
+                const int fr = fail_rows(Wc);
+                if (fr) atomicAdd(&cnt[f], (uint32_t)fr);
+            }
+        }
+    };
+    if constexpr (W2) {
+        const int w = __builtin_amdgcn_readfirstlane(tid) >> 6;
+        static_for<0, Z / 32>([&](auto Wc) {
+            if (w == decltype(Wc)::value) run(Wc);
+        });
+    } else {
+        run(std::integral_constant<int, -1>{});
     }
     const bool converged = a.early_stop && done && active;
     if (!converged) it_done = a.max_iter;
 
-    __syncthreads();
-    if (a.synd || a.ret) {
-        if (active) {
-            const int fr = fail_rows();
-            if (fr) atomicAdd(&cnt[f], (uint32_t)fr);
-        }
-    }
     __syncthreads();
     if (active && r == 0) {
         const unsigned long long g = gframe(f);
@@ -465,7 +535,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     if (a.post) {
         for (int s = 0; s < Feff; s++) {
             f32x4* dst = reinterpret_cast<f32x4*>(a.post + goff(s));
-            const f32x4* src = reinterpret_cast<const f32x4*>(rxall + s * N);
+            const f32x4* src = reinterpret_cast<const f32x4*>(rxall + s * NS);
             for (int i = tid; i < N / 4; i += blockDim.x) __builtin_nontemporal_store(src[i], dst + i);
         }
     }
@@ -477,7 +547,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         for (int q = tid >> 6; q < Feff * kChunks; q += nw) {
             const int s = q / kChunks, c0 = (q - s * kChunks) * 64;
             const int v = c0 + lane;
-            const bool bit = v < N && rxall[s * N + v] >= 0.0f;
+            const bool bit = v < N && rxall[s * NS + v] >= 0.0f;
             const unsigned long long m = __ballot(bit);
             const int byte = (c0 >> 3) + lane;
             if (lane < 8 && byte < N / 8) a.hard[goff(s) / 8 + byte] = (uint8_t)(m >> (8 * lane));
@@ -581,8 +651,9 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
 template <int RATE, int Z, int MODE, int VAR = 0, bool EXT = false>
 int occupancy_tdmp(int F)
 {
+    if ((VAR & VAR_W2) != 0 && F != 2) return 0;
     const int threads = ((F * Z + 63) / 64) * 64;
-    const size_t lds = kLdsPad + (size_t)F * Code<RATE, Z>::N * sizeof(float);
+    const size_t lds = kLdsPad + (size_t)F * lds_frame_stride<Code<RATE, Z>::N, VAR>() * sizeof(float);
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tdmp_kernel<RATE, Z, MODE, VAR, EXT>, threads, lds) !=
         hipSuccess)
@@ -597,9 +668,10 @@ hipError_t launch_tdmp(const DecodeArgs& a, hipStream_t stream)
 {
     const int F = a.frames_per_wg;
     if (F < 1 || F > max_frames_per_wg<RATE, Z>()) return hipErrorInvalidValue;
+    if ((VAR & VAR_W2) != 0 && F != 2) return hipErrorInvalidValue;   // the two-halves lane mapping
     if (a.nframes == 0) return hipSuccess;
     const int threads = ((F * Z + 63) / 64) * 64;
-    const size_t lds = kLdsPad + (size_t)F * Code<RATE, Z>::N * sizeof(float);
+    const size_t lds = kLdsPad + (size_t)F * lds_frame_stride<Code<RATE, Z>::N, VAR>() * sizeof(float);
     unsigned long long grid = (a.nframes + F - 1) / F;
     if (a.tile_counter && a.persistent_grid > 0 && (unsigned long long)a.persistent_grid < grid)
         grid = (unsigned long long)a.persistent_grid;
