@@ -9,7 +9,7 @@ per-frame status in HBM).  The CPU baseline is the oracle (the C restatement
 of the reference decoder, OpenMP over frames as lib/decode_example.cc does)
 on the same LLR buffer, timed on this host's cores.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 20] [--no-cpu]
     python bench.py --workload small   # rate 1/2, z = 24, 1024 frames at 2.5 dB,
                                        # max 10 iterations with early stop, vs the CPU decoder
 """
@@ -34,8 +34,8 @@ def log(*a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--frames", type=int, default=FRAMES)
     ap.add_argument("--frames-per-wg", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
