@@ -49,6 +49,17 @@ constexpr DecodeOccupancy w2_occupancy()
     else return nullptr;
 }
 
+// learned inference: the two-halves shape where the decoder runs two frames
+// per workgroup and z % 32 == 0 (as the normalised min-sum kernel tuned
+// there), the hoisted-address shape otherwise
+template <int RATE, int Z>
+hipError_t launch_learned(const DecodeArgs& a, hipStream_t stream)
+{
+    if constexpr (Z % 32 == 0)
+        if (a.frames_per_wg == 2) return launch_tdmp<RATE, Z, MODE_LEARNED, VAR_LEAN_W2>(a, stream);
+    return launch_tdmp<RATE, Z, MODE_LEARNED, 0>(a, stream);
+}
+
 template <int RATE, int Z>
 constexpr DecodeOccupancy fb_occupancy()
 {
@@ -86,7 +97,7 @@ constexpr auto make_decode_table(std::integer_sequence<int, ZI...>)
     return Arr{{DecodeEntry{{&launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0>,
                              &launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN>,
                              w2_launcher<RATE, z_of_index(ZI)>()},
-                            &launch_tdmp<RATE, z_of_index(ZI), MODE_LEARNED, 0>,
+                            &launch_learned<RATE, z_of_index(ZI)>,
                             max_frames_per_wg<RATE, z_of_index(ZI)>(),
                             {&occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0, true>,
                              &occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN, true>,
