@@ -84,12 +84,14 @@ def test_training_reduces_loss_and_resumes(gpu, oracle_mod, tmp_path):
     assert la == lb and np.array_equal(tr.offsets(), tr2.offsets())   # resumed run is identical
 
 
-@pytest.mark.parametrize("z,order", [(96, "split"), (96, "interleaved"), (48, "split"), (24, "interleaved")])
+@pytest.mark.parametrize("z,order", [(96, "split"), (96, "interleaved"), (48, "split"), (24, "interleaved"),
+                                     (32, "interleaved")])
 def test_fallback_reroutes_failed_frames(gpu, oracle_mod, z, order):
     """Frames the learned pass cannot clean (pure noise) are decoded again by
     plain normalised min-sum from their input LLRs; clean ones are not.
-    z = 24 / 96: one persistent kernel re-queueing in LDS; z = 48: learned
-    pass, compaction, normalised min-sum pass."""
+    z = 24 / 96: one persistent kernel re-queueing in LDS; z = 48 / 32:
+    learned pass, compaction, normalised min-sum pass over the compacted
+    frame list (at z = 32 with the two-halves code shape the table picks)."""
     import wimax_ldpc_lib_amd as W
     torch = gpu
     code = oracle_mod.Code(1, z)
