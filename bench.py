@@ -44,7 +44,7 @@ def main():
     ap.add_argument("--workload", choices=["headline", "small"], default="headline")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; the decoder shards frames across GPUs "
+        print("bench.py measures one GPU; the decoder shards frames across GPUs "
               "with no exchange, so run one process per GPU (--gpus 1).", file=sys.stderr)
         sys.exit(2)
 

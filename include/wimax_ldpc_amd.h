@@ -35,7 +35,7 @@ extern "C" {
 #define WL_EINVAL  (-22)   /* bad rate, z, size or pointer */
 #define WL_ENOMEM  (-12)   /* device or host allocation failed */
 #define WL_EDEVICE  (-5)   /* HIP runtime error (no device, launch failure, ...) */
-#define WL_ENOTSUP (-95)   /* valid request this build does not support */
+#define WL_ENOTSUP (-95)   /* valid request the library does not support */
 
 /* coderate values, identical to the reference's enum coderate (include/enums.h:4-12) */
 #define WL_RATE_1_2   0
@@ -102,7 +102,7 @@ int wl_decoder_get_frames_per_wg(const wl_decoder* dec, int* out);
 int wl_decoder_set_variant(wl_decoder* dec, int variant);
 int wl_decoder_get_variant(const wl_decoder* dec, int* out);
 
-/* Autotuned launch shape compiled into this build for (rate, z): returns 0,
+/* Autotuned launch shape compiled into the library for (rate, z): returns 0,
  * or 1 when the table has no entry and the heuristic default is reported. */
 int wl_tuned_launch(int rate, int z, int* frames_per_wg, int* variant);
 

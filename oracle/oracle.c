@@ -2,7 +2,7 @@
  * oracle.c -- CPU restatement of the reference codec (TEST INFRASTRUCTURE).
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use
- * this file, as the checker; the product never calls it.  Build flags keep
+ * this file, to check results against; the product never calls it.  Build flags keep
  * IEEE semantics (no fast-math, no FMA contraction) so that the double-
  * precision message step rounds exactly as the reference's C++ expression.
  * Reference paths are relative to the reference repository root.

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (wimax_ldpc_lib_amd/)
  * links, loads or calls this code; only tests/, __graft_entry__.smoke() and
- * bench.py's CPU-baseline leg use it, as the checker.
+ * bench.py's CPU-baseline leg use it, to check results against.
  *
  * Every function restates a reference routine (paths relative to the
  * reference repository root):

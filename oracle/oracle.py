@@ -2,7 +2,7 @@
 reference build in oracle/_ref -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
-module, as the checker.  Function-level reference citations are in oracle.c.
+module, to check results against.  Function-level reference citations are in oracle.c.
 """
 import ctypes as C
 import os

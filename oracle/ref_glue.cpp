@@ -2,7 +2,7 @@
 // (lib/ldpc.cc, lib/matrices/*.cc, lib/ldpc_encoder.cc), for tests only.
 // Built by oracle/Makefile into oracle/_ref/ from the sources where they lie;
 // the reference decoder (lib/ldpc_decoder.cc) needs VOLK, which this image
-// lacks, so it is not part of this build.
+// lacks, so it is not compiled here.
 #include <stdint.h>
 #include <string.h>
 #include <exception>
