@@ -94,9 +94,10 @@ int wl_decoder_destroy(wl_decoder* dec);
  * normalised min-sum kernel (0: slot addresses held in registers, 1: slot
  * addresses recomputed per layer, fewer registers, 2: as 1 with each wave
  * holding 32 rows of two frames -- only for z % 32 == 0 and
- * frames_per_wg 2, set first; setting another frames_per_wg then switches
- * to 1).  Results are identical for every shape; create() starts from the
- * autotuned table. */
+ * frames_per_wg 2, 3: 16 rows of four frames -- only for z = 32, 48, 64 and
+ * frames_per_wg 4; set frames_per_wg first, and another frames_per_wg then
+ * switches 2 and 3 to 1).  Results are identical for every shape; create()
+ * starts from the autotuned table. */
 int wl_decoder_set_frames_per_wg(wl_decoder* dec, int frames_per_wg);
 int wl_decoder_get_frames_per_wg(const wl_decoder* dec, int* out);
 int wl_decoder_set_variant(wl_decoder* dec, int variant);

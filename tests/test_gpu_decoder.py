@@ -225,3 +225,32 @@ def test_two_halves_code_shape(gpu, oracle_mod, rate, z):
         assert dec.variant == 1
     with pytest.raises(W.WLError):
         W.ldpc_decoder(0, 24, 5).variant = 2      # z % 32 != 0: no such shape
+
+
+@pytest.mark.parametrize("rate,z", [(5, 48), (0, 64), (2, 32), (4, 48)])
+def test_quarters_code_shape(gpu, oracle_mod, rate, z):
+    """Code shape 3 (each wave holds 16 rows of four frames, per-wave
+    compile-time slot addressing) equals the oracle in both stop modes, with a
+    partial last workgroup (35 frames = 8 x 4 + 3); it needs four frames per
+    workgroup, and another value falls back to shape 1."""
+    import wimax_ldpc_lib_amd as W
+    code = oracle_mod.Code(rate, z)
+    if rate == 4:
+        llr = np.random.default_rng(z).normal(0.8, 2.0, (35, code.n)).astype(np.float32)
+    else:
+        llr, _ = noisy_llr(code, 35, {0: 1.8, 2: 2.4, 5: 3.4}[rate], seed=700 + z)
+    for early in (True, False):
+        o = code.decode(llr, max_iter=9, early_stop=early)
+        dec = W.ldpc_decoder(rate, z, 9, early_stop=early)
+        dec.variant = 0
+        dec.frames_per_wg = 2
+        with pytest.raises(W.WLError):
+            dec.variant = 3
+        dec.frames_per_wg = 4
+        dec.variant = 3
+        assert dec.variant == 3
+        _compare(_gpu_decode(gpu, dec, llr), o, "shape 3 rate %d z %d early %d" % (rate, z, early))
+        dec.frames_per_wg = 3
+        assert dec.variant == 1
+    with pytest.raises(W.WLError):
+        W.ldpc_decoder(5, 96, 5).variant = 3      # more than four waves: no such shape

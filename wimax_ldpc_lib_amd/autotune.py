@@ -3,7 +3,8 @@
 For every code it times the normalised min-sum kernel (exactly `--iters`
 iterations, no early stop, so the work is the same for every shape) over
 each code-shape variant and every frames-per-workgroup value the kernel
-accepts (shape 2, where z % 32 == 0, at two frames per workgroup only),
+accepts (shape 2, where z % 32 == 0, at two frames per workgroup only;
+shape 3, at z = 32, 48, 64, at four),
 on a batch of seeded channel frames holding about 150 Mbit of
 codeword bits, and keeps the fastest.  Every shape is checked to give the
 same hard decisions as the first one.  Results go to
@@ -45,6 +46,8 @@ def tune_one(torch, W, rate, z, iters, reps, bits_target):
     shapes = [(v, f) for v in (0, 1) for f in range(1, fmax + 1)]
     if z % 32 == 0 and fmax >= 2:
         shapes.append((2, 2))   # two-halves lane mapping: two frames per workgroup only
+    if z % 16 == 0 and z <= 64 and fmax >= 4:
+        shapes.append((3, 4))   # quarters lane mapping: four frames per workgroup only
     for variant, f in shapes:
         dec.variant = 0
         dec.frames_per_wg = f

@@ -269,7 +269,7 @@ int wl_decoder_create(int rate, int z, int max_iter, unsigned flags, wl_decoder*
     d->entry = wl::decode_entry(rate, z);
     int tf = 0, tv = 0;
     if (tuned_lookup(rate, z, &tf, &tv) && tf >= 1 && tf <= d->entry->fmax && tv >= 0 && tv < wl::kNumVariants &&
-        d->entry->nms[tv] && (tv != wl::kVariantW2 || tf == 2)) {
+        d->entry->nms[tv] && (wl::variant_frames(tv) == 0 || tf == wl::variant_frames(tv))) {
         d->frames_per_wg = tf;
         d->variant = tv;
     } else {
@@ -306,9 +306,11 @@ int wl_decoder_set_variant(wl_decoder* dec, int variant)
     if (dec->generic) return variant == 0 ? WL_OK : fail(WL_EINVAL, "run-time codes have one code shape");
     if (variant < 0 || variant >= wl::kNumVariants)
         return fail(WL_EINVAL, "variant %d out of range 0..%d", variant, wl::kNumVariants - 1);
-    if (!dec->entry->nms[variant]) return fail(WL_EINVAL, "variant %d needs z %% 32 == 0", variant);
-    if (variant == wl::kVariantW2 && dec->frames_per_wg != 2)
-        return fail(WL_EINVAL, "variant %d decodes two frames per workgroup (set frames_per_wg to 2 first)", variant);
+    if (!dec->entry->nms[variant]) return fail(WL_EINVAL, "variant %d does not exist at this z", variant);
+    const int vf = wl::variant_frames(variant);
+    if (vf && dec->frames_per_wg != vf)
+        return fail(WL_EINVAL, "variant %d decodes %d frames per workgroup (set frames_per_wg to %d first)", variant,
+                    vf, vf);
     dec->variant = variant;
     return WL_OK;
 }
@@ -331,7 +333,8 @@ int wl_decoder_set_frames_per_wg(wl_decoder* dec, int f)
     }
     if (f < 1 || f > dec->entry->fmax)
         return fail(WL_EINVAL, "frames_per_wg %d out of range 1..%d", f, dec->entry->fmax);
-    if (dec->variant == wl::kVariantW2 && f != 2) dec->variant = 1;   // same shape without the two-halves mapping
+    const int vf = wl::variant_frames(dec->variant);
+    if (vf && f != vf) dec->variant = 1;   // the same shape without the wave-split lane mapping
     dec->frames_per_wg = f;
     return WL_OK;
 }

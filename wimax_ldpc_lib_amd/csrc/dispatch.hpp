@@ -13,9 +13,17 @@ using DecodeOccupancy = int (*)(int frames_per_wg);
 
 // code-shape variants of the NMS kernel (tdmp.hpp VAR_*): 0 hoisted row
 // addresses, 1 VAR_LEAN, 2 VAR_LEAN_W2 (two frames per workgroup, z % 32 == 0
-// only; null elsewhere)
-constexpr int kNumVariants = 3;
+// only), 3 VAR_LEAN_W4 (four frames per workgroup, z = 32, 48, 64 only: one
+// loop copy per wave, at most four); null where a shape does not exist
+constexpr int kNumVariants = 4;
 constexpr int kVariantW2 = 2;
+constexpr int kVariantW4 = 3;
+
+// frames per workgroup a variant requires (0: any)
+constexpr int variant_frames(int variant)
+{
+    return variant == kVariantW2 ? 2 : (variant == kVariantW4 ? 4 : 0);
+}
 
 struct DecodeEntry {
     DecodeLaunch nms[kNumVariants];  // normalised min-sum, the reference's arithmetic
@@ -46,6 +54,20 @@ template <int RATE, int Z>
 constexpr DecodeOccupancy w2_occupancy()
 {
     if constexpr (Z % 32 == 0) return &occupancy_tdmp<RATE, Z, MODE_NMS, VAR_LEAN_W2, true>;
+    else return nullptr;
+}
+
+template <int RATE, int Z>
+constexpr DecodeLaunch w4_launcher()
+{
+    if constexpr (Z % 16 == 0 && Z <= 64) return &launch_tdmp<RATE, Z, MODE_NMS, VAR_LEAN_W4>;
+    else return nullptr;
+}
+
+template <int RATE, int Z>
+constexpr DecodeOccupancy w4_occupancy()
+{
+    if constexpr (Z % 16 == 0 && Z <= 64) return &occupancy_tdmp<RATE, Z, MODE_NMS, VAR_LEAN_W4, true>;
     else return nullptr;
 }
 
@@ -96,12 +118,12 @@ constexpr auto make_decode_table(std::integer_sequence<int, ZI...>)
     struct Arr { DecodeEntry e[sizeof...(ZI)]; };
     return Arr{{DecodeEntry{{&launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0>,
                              &launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN>,
-                             w2_launcher<RATE, z_of_index(ZI)>()},
+                             w2_launcher<RATE, z_of_index(ZI)>(), w4_launcher<RATE, z_of_index(ZI)>()},
                             &launch_learned<RATE, z_of_index(ZI)>,
                             max_frames_per_wg<RATE, z_of_index(ZI)>(),
                             {&occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0, true>,
                              &occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN, true>,
-                             w2_occupancy<RATE, z_of_index(ZI)>()},
+                             w2_occupancy<RATE, z_of_index(ZI)>(), w4_occupancy<RATE, z_of_index(ZI)>()},
                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_LEARNED, 0, true>,
                             fb_launcher<RATE, z_of_index(ZI)>(),
                             fb_occupancy<RATE, z_of_index(ZI)>()}...}};

@@ -33,7 +33,7 @@ struct GroupShape {
     // which fixes F = 2 (if all were, F = 2)
     static constexpr int rate_f(int r)
     {
-        return tuned_variant(r, Z) == 2 ? 0 : (tuned_f(r, Z) > 0 ? tuned_f(r, Z) : 1);
+        return tuned_variant(r, Z) >= 2 ? 0 : (tuned_f(r, Z) > 0 ? tuned_f(r, Z) : 1);
     }
     static constexpr int pick_f()
     {

@@ -141,16 +141,28 @@ enum { MODE_NMS = 0, MODE_LEARNED = 1, MODE_LEARNED_FB = 2 };
 //                one wave's 32 rows, so the other waves address it without a
 //                select.  Frames sit 32 floats apart beyond n in LDS so the
 //                two halves of a wave read different banks.
-enum { VAR_NOHOIST = 1, VAR_REREAD = 2, VAR_SEL = 4, VAR_SMASK = 8, VAR_W2 = 16 };
+//   VAR_W4       the same with four frames per workgroup and z % 16 == 0:
+//                lane l of wave w owns row 16 w + l % 16 of frame l / 16
+//                (frames 16 floats apart beyond n)
+enum { VAR_NOHOIST = 1, VAR_REREAD = 2, VAR_SEL = 4, VAR_SMASK = 8, VAR_W2 = 16, VAR_W4 = 32 };
 // the low-register code shape of the tuned variants
 constexpr int VAR_LEAN = VAR_NOHOIST | VAR_SEL | VAR_SMASK;
 constexpr int VAR_LEAN_W2 = VAR_LEAN | VAR_W2;
+constexpr int VAR_LEAN_W4 = VAR_LEAN | VAR_W4;
+
+// rows of one frame per wave in the wave-split shapes (0: not split); the
+// workgroup then decodes 64 / rows frames
+template <int VAR>
+constexpr int split_rows()
+{
+    return (VAR & VAR_W2) != 0 ? 32 : ((VAR & VAR_W4) != 0 ? 16 : 0);
+}
 
 // floats between consecutive frames in LDS
 template <int N, int VAR>
 constexpr int lds_frame_stride()
 {
-    return (VAR & VAR_W2) != 0 ? N + 32 : N;
+    return N + split_rows<VAR>();
 }
 
 template <int RATE, int Z>
@@ -176,8 +188,9 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     constexpr int NL = CD::NL;
     constexpr int N = CD::N;
     constexpr int C = CD::C;
-    constexpr bool W2 = (VAR & VAR_W2) != 0;
-    static_assert(!W2 || (Z % 32 == 0 && (VAR & VAR_SEL) != 0), "VAR_W2 needs z % 32 == 0 and VAR_SEL");
+    constexpr int RS = split_rows<VAR>();   // wave-split shapes: rows of one frame per wave
+    constexpr bool W2 = RS != 0;             // (either split shape)
+    static_assert(!W2 || (Z % RS == 0 && (VAR & VAR_SEL) != 0), "wave-split shapes need z % rows == 0 and VAR_SEL");
     constexpr int NS = lds_frame_stride<N, VAR>();
 
     uint32_t* cnt = reinterpret_cast<uint32_t*>(smem);
@@ -187,8 +200,8 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     const int F = a.frames_per_wg;
     int f, r;
     if constexpr (W2) {
-        f = (tid >> 5) & 1;
-        r = ((tid >> 6) << 5) + (tid & 31);
+        f = (tid & 63) / RS;
+        r = (tid >> 6) * RS + (tid & (RS - 1));
     } else {
         f = tid / Z;
         r = tid - f * Z;
@@ -285,9 +298,10 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
 
     // LDS byte address of slot (L, J) of this lane's row: one of the two row
     // bases (ob0: rows r, obW: r - Z) plus the slot's constant offset.  With
-    // VAR_W2, W is this wave's index (rows 32 W .. 32 W + 31): where all of
-    // them wrap or none does, the base is chosen at compile time; otherwise
-    // (and without VAR_W2) one select on the lane mask.
+    // a wave-split shape, W is this wave's index (rows RS W .. RS W + RS - 1):
+    // where all of them wrap or none does, the base is chosen at compile
+    // time; otherwise (and without a wave-split shape) one select on the
+    // lane mask.
     auto sel_addr = [&](auto Lc2, auto Jc2, auto Wc, uint32_t ob0, uint32_t obW, cmask_t* mt) -> uint32_t {
         constexpr int L2 = decltype(Lc2)::value;
         constexpr int J2 = decltype(Jc2)::value;
@@ -297,14 +311,17 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         if constexpr (s == 0) {
             return ob0 + off;
         } else if constexpr (W2) {
-            constexpr int lo = 32 * decltype(Wc)::value, bnd = Z - s;   // rows >= bnd wrap
-            if constexpr (bnd >= lo + 32) {
+            constexpr int lo = RS * decltype(Wc)::value, bnd = Z - s;   // rows >= bnd wrap
+            if constexpr (bnd >= lo + RS) {
                 return ob0 + off;
             } else if constexpr (bnd <= lo) {
                 return obW + off;
             } else {
-                constexpr uint32_t m32 = ~0u << (bnd - lo);
-                uint64_t m = (uint64_t)m32 | ((uint64_t)m32 << 32);
+                // lanes whose row (l % RS) + lo >= bnd, in each of the 64 / RS frames
+                constexpr uint64_t seg = ((RS == 32 ? 0xffffffffull : 0xffffull) << (bnd - lo)) &
+                                         (RS == 32 ? 0xffffffffull : 0xffffull);
+                uint64_t m = 0;
+                for (int q = 0; q < 64 / RS; q++) m |= seg << (q * RS);
                 // in the extended kernels (more scalar state) materialised
                 // here, two scalar moves, rather than hoisted and spilled
                 if constexpr (EXT) asm volatile("" : "+s"(m));
@@ -467,7 +484,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     bool done = !active;
     int it_done = 0;
     // Initial check, iterations and final syndrome.  W: this wave's index
-    // with VAR_W2 (each wave runs its own copy; every copy passes the same
+    // with a wave-split shape (each wave runs its own copy; every copy passes the same
     // barriers), -1 otherwise.
     auto run = [&](auto Wc) {
         if (a.early_stop) {
@@ -506,7 +523,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     };
     if constexpr (W2) {
         const int w = __builtin_amdgcn_readfirstlane(tid) >> 6;
-        static_for<0, Z / 32>([&](auto Wc) {
+        static_for<0, Z / RS>([&](auto Wc) {
             if (w == decltype(Wc)::value) run(Wc);
         });
     } else {
@@ -651,7 +668,7 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_kernel(DecodeArgs a)
 template <int RATE, int Z, int MODE, int VAR = 0, bool EXT = false>
 int occupancy_tdmp(int F)
 {
-    if ((VAR & VAR_W2) != 0 && F != 2) return 0;
+    if (split_rows<VAR>() != 0 && F != 64 / split_rows<VAR>()) return 0;
     const int threads = ((F * Z + 63) / 64) * 64;
     const size_t lds = kLdsPad + (size_t)F * lds_frame_stride<Code<RATE, Z>::N, VAR>() * sizeof(float);
     int nb = 0;
@@ -668,7 +685,7 @@ hipError_t launch_tdmp(const DecodeArgs& a, hipStream_t stream)
 {
     const int F = a.frames_per_wg;
     if (F < 1 || F > max_frames_per_wg<RATE, Z>()) return hipErrorInvalidValue;
-    if ((VAR & VAR_W2) != 0 && F != 2) return hipErrorInvalidValue;   // the two-halves lane mapping
+    if (split_rows<VAR>() != 0 && F != 64 / split_rows<VAR>()) return hipErrorInvalidValue;   // wave-split lane mapping
     if (a.nframes == 0) return hipSuccess;
     const int threads = ((F * Z + 63) / 64) * 64;
     const size_t lds = kLdsPad + (size_t)F * lds_frame_stride<Code<RATE, Z>::N, VAR>() * sizeof(float);
