@@ -332,16 +332,24 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         }
     };
 
+    // The row as a register value is only needed by the compare-based slot
+    // addressing (shapes without a mask table); the mask-table and wave-split
+    // shapes address slots from the two row bases alone.
+    constexpr bool uses_rr = ((VAR & VAR_SEL) != 0) ? ((VAR & VAR_SMASK) == 0 && !W2) : ((VAR & VAR_NOHOIST) != 0);
+    // VAR_SEL: LDS byte addresses of the two row bases, made opaque again at
+    // every layer; each layer's asm consumes the previous layer's value, so no
+    // copy is kept live across the loop
+    uint32_t lob0 = lds_addr(smem) + (uint32_t)(kLdsPad + ((active ? f : 0) * NS + r) * 4);
+    uint32_t lobW = lob0 - (uint32_t)(Z * 4);
+
     // One layer for this lane's row (lib/ldpc_decoder.cc:143-240).
     auto layer = [&](auto Lc, int t, auto Wc) {
         constexpr int L = decltype(Lc)::value;
         constexpr int D = CD::deg(L);
         int rr = r;
-        if constexpr ((VAR & VAR_NOHOIST) != 0) asm volatile("" : "+v"(rr));
-        // VAR_SEL: LDS byte addresses of the two row bases, opaque per layer
-        uint32_t ob0 = lds_addr(smem) + (uint32_t)(kLdsPad + ((active ? f : 0) * NS + r) * 4);
-        uint32_t obW = ob0 - (uint32_t)(Z * 4);
-        if constexpr ((VAR & VAR_SEL) != 0) asm volatile("" : "+v"(ob0), "+v"(obW));
+        if constexpr (uses_rr) asm volatile("" : "+v"(rr));
+        if constexpr ((VAR & VAR_SEL) != 0) asm volatile("" : "+v"(lob0), "+v"(lobW));
+        const uint32_t ob0 = lob0, obW = lobW;
         // with one wave phase (z = 32, 64) the masks are constants; opaque,
         // they stay per-layer scalar loads rather than registers held (and,
         // in the larger kernels, spilled) across the iteration loop
@@ -457,7 +465,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         static_for<0, NL>([&](auto Lc) {
             constexpr int L = decltype(Lc)::value;
             int rr = r;
-            if constexpr ((VAR & VAR_NOHOIST) != 0) asm volatile("" : "+v"(rr));
+            if constexpr (uses_rr) asm volatile("" : "+v"(rr));
             const uint32_t ob0 = lds_addr(smem) + (uint32_t)(kLdsPad + ((active ? f : 0) * NS + r) * 4);
             const uint32_t obW = ob0 - (uint32_t)(Z * 4);
             cmask_t* mt = (cmask_t*)&WrapMasks<RATE, Z>::table.m[wphase][L][0];
