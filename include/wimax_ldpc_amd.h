@@ -90,7 +90,8 @@ int wl_decoder_destroy(wl_decoder* dec);
 
 /* Launch shape of a decoder (no counterpart in the reference, whose decoder
  * is one frame per CPU thread).  frames_per_wg: frames decoded together by
- * one workgroup (0 = the autotuned default); variant: code shape of the
+ * one workgroup (0 = the autotuned launch shape, code shape included);
+ * variant: code shape of the
  * normalised min-sum kernel (0: slot addresses held in registers, 1: slot
  * addresses recomputed per layer, fewer registers, 2: as 1 with each wave
  * holding 32 rows of two frames -- only for z % 32 == 0 and

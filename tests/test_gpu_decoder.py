@@ -223,6 +223,9 @@ def test_two_halves_code_shape(gpu, oracle_mod, rate, z):
         _compare(_gpu_decode(gpu, dec, llr), o, "shape 2 rate %d z %d early %d" % (rate, z, early))
         dec.frames_per_wg = 1
         assert dec.variant == 1
+        fresh = W.ldpc_decoder(rate, z, 9, early_stop=early)
+        dec.frames_per_wg = 0                     # back to the autotuned launch shape
+        assert (dec.frames_per_wg, dec.variant) == (fresh.frames_per_wg, fresh.variant)
     with pytest.raises(W.WLError):
         W.ldpc_decoder(0, 24, 5).variant = 2      # z % 32 != 0: no such shape
 

@@ -327,9 +327,18 @@ int wl_decoder_set_frames_per_wg(wl_decoder* dec, int f)
     if (!dec) return fail(WL_EINVAL, "null decoder");
     if (dec->generic) return (f == 0 || f == 1) ? WL_OK : fail(WL_EINVAL, "run-time codes decode one frame per workgroup");
     if (f == 0) {
-        int tv = 0;
-        if (!tuned_lookup(dec->code.info.rate, dec->code.info.z, &f, &tv))
-            f = default_frames_per_wg(dec->code.info.z, dec->entry->fmax);
+        // the autotuned launch shape: frames per workgroup and code shape
+        int tf = 0, tv = 0;
+        if (tuned_lookup(dec->code.info.rate, dec->code.info.z, &tf, &tv) && tf >= 1 && tf <= dec->entry->fmax &&
+            tv >= 0 && tv < wl::kNumVariants && dec->entry->nms[tv] &&
+            (wl::variant_frames(tv) == 0 || tf == wl::variant_frames(tv))) {
+            dec->frames_per_wg = tf;
+            dec->variant = tv;
+        } else {
+            dec->frames_per_wg = default_frames_per_wg(dec->code.info.z, dec->entry->fmax);
+            dec->variant = 0;
+        }
+        return WL_OK;
     }
     if (f < 1 || f > dec->entry->fmax)
         return fail(WL_EINVAL, "frames_per_wg %d out of range 1..%d", f, dec->entry->fmax);

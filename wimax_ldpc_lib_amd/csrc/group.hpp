@@ -29,8 +29,8 @@ struct GroupArgs {
 // rates, ties to the smaller block.
 template <int Z>
 struct GroupShape {
-    // a rate's tuned F counts unless it was tuned to the two-halves shape,
-    // which fixes F = 2 (if all were, F = 2)
+    // a rate's tuned F counts unless it was tuned to a wave-split shape (2
+    // or 3), whose F is fixed by the shape (if all were, F = 2)
     static constexpr int rate_f(int r)
     {
         return tuned_variant(r, Z) >= 2 ? 0 : (tuned_f(r, Z) > 0 ? tuned_f(r, Z) : 1);

@@ -189,8 +189,8 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     constexpr int N = CD::N;
     constexpr int C = CD::C;
     constexpr int RS = split_rows<VAR>();   // wave-split shapes: rows of one frame per wave
-    constexpr bool W2 = RS != 0;             // (either split shape)
-    static_assert(!W2 || (Z % RS == 0 && (VAR & VAR_SEL) != 0), "wave-split shapes need z % rows == 0 and VAR_SEL");
+    constexpr bool SPLIT = RS != 0;
+    static_assert(!SPLIT || (Z % RS == 0 && (VAR & VAR_SEL) != 0), "wave-split shapes need z % rows == 0 and VAR_SEL");
     constexpr int NS = lds_frame_stride<N, VAR>();
 
     uint32_t* cnt = reinterpret_cast<uint32_t*>(smem);
@@ -199,7 +199,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     const int tid = threadIdx.x;
     const int F = a.frames_per_wg;
     int f, r;
-    if constexpr (W2) {
+    if constexpr (SPLIT) {
         f = (tid & 63) / RS;
         r = (tid >> 6) * RS + (tid & (RS - 1));
     } else {
@@ -310,7 +310,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         constexpr uint32_t off = (uint32_t)((b * Z + s) * 4);
         if constexpr (s == 0) {
             return ob0 + off;
-        } else if constexpr (W2) {
+        } else if constexpr (SPLIT) {
             constexpr int lo = RS * decltype(Wc)::value, bnd = Z - s;   // rows >= bnd wrap
             if constexpr (bnd >= lo + RS) {
                 return ob0 + off;
@@ -335,7 +335,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     // The row as a register value is only needed by the compare-based slot
     // addressing (shapes without a mask table); the mask-table and wave-split
     // shapes address slots from the two row bases alone.
-    constexpr bool uses_rr = ((VAR & VAR_SEL) != 0) ? ((VAR & VAR_SMASK) == 0 && !W2) : ((VAR & VAR_NOHOIST) != 0);
+    constexpr bool uses_rr = ((VAR & VAR_SEL) != 0) ? ((VAR & VAR_SMASK) == 0 && !SPLIT) : ((VAR & VAR_NOHOIST) != 0);
     // VAR_SEL: LDS byte addresses of the two row bases, made opaque again at
     // every layer; each layer's asm consumes the previous layer's value, so no
     // copy is kept live across the loop
@@ -356,7 +356,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         cmask_t* mtl = (cmask_t*)&WrapMasks<RATE, Z>::table.m[wphase][L][0];
         if constexpr ((VAR & VAR_SMASK) != 0 && WrapMasks<RATE, Z>::P == 1) asm volatile("" : "+s"(mtl));
         auto atl = [&](auto Lc2, auto Jc2) {
-            if constexpr ((VAR & VAR_SEL) != 0 && ((VAR & VAR_SMASK) != 0 || W2)) {
+            if constexpr ((VAR & VAR_SEL) != 0 && ((VAR & VAR_SMASK) != 0 || SPLIT)) {
                 return lds_ptr(sel_addr(Lc2, Jc2, Wc, ob0, obW, mtl));
             } else if constexpr ((VAR & VAR_SEL) != 0) {
                 constexpr int b = CD::blk(L, decltype(Jc2)::value);
@@ -474,7 +474,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             static_for<0, CD::deg(L)>([&](auto Jc) {
                 constexpr int J = decltype(Jc)::value;
                 float xv;
-                if constexpr ((VAR & VAR_SEL) != 0 && ((VAR & VAR_SMASK) != 0 || W2)) {
+                if constexpr ((VAR & VAR_SEL) != 0 && ((VAR & VAR_SMASK) != 0 || SPLIT)) {
                     xv = *lds_ptr(sel_addr(Lc, Jc, Wc, ob0, obW, mt));
                 } else if constexpr ((VAR & VAR_NOHOIST) != 0) {
                     xv = *at_r(Lc, Jc, rr);
@@ -529,7 +529,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             }
         }
     };
-    if constexpr (W2) {
+    if constexpr (SPLIT) {
         const int w = __builtin_amdgcn_readfirstlane(tid) >> 6;
         static_for<0, Z / RS>([&](auto Wc) {
             if (w == decltype(Wc)::value) run(Wc);
