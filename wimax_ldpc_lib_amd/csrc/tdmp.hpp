@@ -109,7 +109,8 @@ __device__ __forceinline__ uint32_t select_by_mask(uint32_t a, uint32_t b, uint6
     return __builtin_amdgcn_inverse_ballot_w64(mask) ? b : a;
 }
 
-constexpr int kLdsPad = 512;   // per-frame counters; also keeps wrapped row bases >= 0
+constexpr int kLdsPad = 768;   // per-frame counters and offsets; also keeps wrapped row bases >= 0
+constexpr int kFrameOffOff = 512;   // extended tiles: element offset of each slot's frame (u64 per slot)
 constexpr int kTileSlot = 504; // persistent mode: the tile index broadcast, inside the pad
 constexpr int kQueueOff = 128;      // MODE_LEARNED_FB: re-queue of failed frame ids (2F uint32)
 constexpr int kFailOff = 304;       // MODE_LEARNED_FB: per-slot return values (F bytes)
@@ -228,6 +229,20 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     if constexpr (EXT) contiguous = !a.frame_idx && !a.frame_off && !ids;
 
     if (tid < F) cnt[tid] = 0;
+    // Extended tiles: each slot's element offset, read once (the frame list
+    // and offset table are a chain of dependent global loads) and kept in
+    // LDS for the frame loads and the output stores.
+    unsigned long long* foff = reinterpret_cast<unsigned long long*>(smem + kFrameOffOff);
+    if constexpr (EXT) {
+        static_assert(kFrameOffOff + 8 * (kMaxThreads / 24) <= kLdsPad, "slot offsets overflow the LDS pad");
+        if (!contiguous && tid < Feff) foff[tid] = goff(tid);
+    }
+    auto fo = [&](int s) -> unsigned long long {
+        if constexpr (EXT) {
+            if (!contiguous) return foff[s];
+        }
+        return (frame0 + s) * N;
+    };
     if (contiguous) {
         const f32x4* src = reinterpret_cast<const f32x4*>(a.llr + frame0 * N);
         f32x4* dst = reinterpret_cast<f32x4*>(rxall);
@@ -253,10 +268,30 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             }
         }
     } else {
-        for (int s = 0; s < Feff; s++) {
-            const f32x4* src = reinterpret_cast<const f32x4*>(a.llr + goff(s));
-            f32x4* dst = reinterpret_cast<f32x4*>(rxall + s * NS);
-            for (int i = tid; i < N / 4; i += blockDim.x) dst[i] = src[i];
+        __syncthreads();   // slot offsets in LDS
+        constexpr int per = N / 4;
+        const int total4 = Feff * per;
+        const int bd = blockDim.x;
+        f32x4* dst = reinterpret_cast<f32x4*>(rxall);
+        // all frames of the tile at once, up to 8 loads per thread in flight
+        for (int i0 = tid; i0 < total4; i0 += 8 * bd) {
+            f32x4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = i0 + u * bd;
+                if (i < total4) {
+                    const int s = i / per;
+                    v[u] = reinterpret_cast<const f32x4*>(a.llr + foff[s])[i - s * per];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = i0 + u * bd;
+                if (i < total4) {
+                    const int s = i / per;
+                    dst[i + s * ((NS - N) / 4)] = v[u];
+                }
+            }
         }
     }
     __syncthreads();
@@ -559,7 +594,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
 
     if (a.post) {
         for (int s = 0; s < Feff; s++) {
-            f32x4* dst = reinterpret_cast<f32x4*>(a.post + goff(s));
+            f32x4* dst = reinterpret_cast<f32x4*>(a.post + fo(s));
             const f32x4* src = reinterpret_cast<const f32x4*>(rxall + s * NS);
             for (int i = tid; i < N / 4; i += blockDim.x) __builtin_nontemporal_store(src[i], dst + i);
         }
@@ -575,7 +610,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             const bool bit = v < N && rxall[s * NS + v] >= 0.0f;
             const unsigned long long m = __ballot(bit);
             const int byte = (c0 >> 3) + lane;
-            if (lane < 8 && byte < N / 8) a.hard[goff(s) / 8 + byte] = (uint8_t)(m >> (8 * lane));
+            if (lane < 8 && byte < N / 8) a.hard[fo(s) / 8 + byte] = (uint8_t)(m >> (8 * lane));
         }
     }
 }
