@@ -61,17 +61,13 @@ def test_mixed_channel_matches_per_code_channel(gpu):
             assert np.array_equal(bits_np[o:o + enc.n], b1[g])
 
 
-def test_mixed_decode_matches_per_code(gpu):
-    """All 114 codes in one batch (3/4B frames carry the reference encoder's
-    bytes, which fail H: they exercise non-converging decodes)."""
+def _check_against_per_code(torch, cfg_np, offs, llr, outs):
+    """Every frame of a mixed decode equals the per-code decoder's result."""
     import wimax_ldpc_lib_amd as W
-    torch = gpu
-    md, cfg_np, cfg, off, offs, n_of, llr, bits = _setup(torch, 6000, 11, list(range(114)))
-    post, hard, ret, iters, synd = _mixed_decode(torch, md, cfg, off, llr)
+    post, hard, ret, iters, synd = outs
     post, hard = post.cpu().numpy(), hard.cpu().numpy()
     ret, iters, synd = ret.cpu().numpy(), iters.cpu().numpy(), synd.cpu().numpy()
     llr_np = llr.cpu().numpy()
-    assert md.invalid_count() == 0
     for c in np.unique(cfg_np):
         rate, z = int(c) // 19, 24 + 4 * (int(c) % 19)
         sel = np.flatnonzero(cfg_np == c)
@@ -94,6 +90,27 @@ def test_mixed_decode_matches_per_code(gpu):
         assert np.array_equal(ret[sel], r1.cpu().numpy())
         assert np.array_equal(iters[sel], i1.cpu().numpy())
         assert np.array_equal(synd[sel], s1.cpu().numpy())
+
+
+def test_mixed_decode_matches_per_code(gpu):
+    """All 114 codes in one batch (3/4B frames carry the reference encoder's
+    bytes, which fail H: they exercise non-converging decodes)."""
+    torch = gpu
+    md, cfg_np, cfg, off, offs, n_of, llr, bits = _setup(torch, 6000, 11, list(range(114)))
+    outs = _mixed_decode(torch, md, cfg, off, llr)
+    assert md.invalid_count() == 0
+    _check_against_per_code(torch, cfg_np, offs, llr, outs)
+
+
+@pytest.mark.parametrize("nframes", [1, 3, 17])
+def test_mixed_tiny_batches_match_per_code(gpu, nframes):
+    """Batches smaller than one tile per code: most codes and lifting-size
+    groups are empty, and the few tiles are partly filled."""
+    torch = gpu
+    md, cfg_np, cfg, off, offs, n_of, llr, bits = _setup(torch, nframes, 40 + nframes, list(range(114)))
+    outs = _mixed_decode(torch, md, cfg, off, llr)
+    assert md.invalid_count() == 0
+    _check_against_per_code(torch, cfg_np, offs, llr, outs)
 
 
 def test_mixed_graph_replays(gpu):
