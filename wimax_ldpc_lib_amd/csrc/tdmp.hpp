@@ -111,6 +111,7 @@ __device__ __forceinline__ uint32_t select_by_mask(uint32_t a, uint32_t b, uint6
 
 constexpr int kLdsPad = 768;   // per-frame counters and offsets; also keeps wrapped row bases >= 0
 constexpr int kFrameOffOff = 512;   // extended tiles: element offset of each slot's frame (u64 per slot)
+constexpr int kFrameIdOff = 680;    // extended tiles: global frame of each slot (u32 per slot)
 constexpr int kTileSlot = 504; // persistent mode: the tile index broadcast, inside the pad
 constexpr int kQueueOff = 128;      // MODE_LEARNED_FB: re-queue of failed frame ids (2F uint32)
 constexpr int kFailOff = 304;       // MODE_LEARNED_FB: per-slot return values (F bytes)
@@ -210,7 +211,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     const bool active = f < Feff;
     // wave phase of the wrap-mask table (uniform over the wave)
     const int wphase = (__builtin_amdgcn_readfirstlane(tid) >> 6) % WrapMasks<RATE, Z>::P;
-    // global frame of slot s, and its element offset
+    // global frame of slot s
     auto gframe = [&](int s) -> unsigned long long {
         if constexpr (EXT) {
             if (ids) return (unsigned long long)ids[s];
@@ -220,22 +221,24 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             return frame0 + s;
         }
     };
-    auto goff = [&](int s) -> unsigned long long {
-        const unsigned long long g = gframe(s);
-        if constexpr (EXT) return a.frame_off ? a.frame_off[g] : g * N;
-        else return g * N;
-    };
     bool contiguous = true;
     if constexpr (EXT) contiguous = !a.frame_idx && !a.frame_off && !ids;
 
     if (tid < F) cnt[tid] = 0;
-    // Extended tiles: each slot's element offset, read once (the frame list
-    // and offset table are a chain of dependent global loads) and kept in
-    // LDS for the frame loads and the output stores.
+    // Extended tiles: each slot's global frame and element offset, read once
+    // (the frame list and offset table are a chain of dependent global
+    // loads) and kept in LDS for the frame loads and the output stores.
     unsigned long long* foff = reinterpret_cast<unsigned long long*>(smem + kFrameOffOff);
+    uint32_t* fid = reinterpret_cast<uint32_t*>(smem + kFrameIdOff);
     if constexpr (EXT) {
-        static_assert(kFrameOffOff + 8 * (kMaxThreads / 24) <= kLdsPad, "slot offsets overflow the LDS pad");
-        if (!contiguous && tid < Feff) foff[tid] = goff(tid);
+        static_assert(kFrameOffOff + 8 * (kMaxThreads / 24) <= kFrameIdOff, "slot offsets overlap the frame ids");
+        static_assert(kFrameIdOff + 4 * (kMaxThreads / 24) <= kLdsPad, "slot frame ids overflow the LDS pad");
+        if (!contiguous && tid < Feff) {
+            // a frame list or queue holds 32-bit frame numbers
+            const unsigned long long g = gframe(tid);
+            fid[tid] = (uint32_t)g;
+            foff[tid] = a.frame_off ? a.frame_off[g] : g * N;
+        }
     }
     auto fo = [&](int s) -> unsigned long long {
         if constexpr (EXT) {
@@ -577,7 +580,10 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
 
     __syncthreads();
     if (active && r == 0) {
-        const unsigned long long g = gframe(f);
+        unsigned long long g = frame0 + f;
+        if constexpr (EXT) {
+            if (!contiguous) g = fid[f];
+        }
         const uint32_t w = cnt[f];
         if (a.synd) a.synd[g] = (uint16_t)w;
         if (a.iters) a.iters[g] = (uint8_t)it_done;
