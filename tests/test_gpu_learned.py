@@ -151,17 +151,18 @@ def test_trained_offsets_beat_untrained(gpu, oracle_mod):
     assert fer["trained"] < 0.9 * fer["untrained"], fer
 
 
-def test_mixed_learned_fallback_matches_per_code(gpu, oracle_mod):
+@pytest.mark.parametrize("F", [2000, 7])
+def test_mixed_learned_fallback_matches_per_code(gpu, oracle_mod, F):
     """All five encoder rates at z = 96 with separate offsets per rate in one
     mixed batch: equal to per-code learned+fallback decodes, fallback frames
-    equal to the oracle's normalised min-sum."""
+    equal to the oracle's normalised min-sum.  The 7-frame batch leaves rates
+    empty and tiles partly filled."""
     import wimax_ldpc_lib_amd as W
     from wimax_ldpc_lib_amd.mixed import MixedDecoder, code_id
     torch = gpu
     rates = [0, 1, 2, 3, 5]
     codes = [code_id(r, 96) for r in rates]
-    F = 2000
-    rng = np.random.default_rng(61)
+    rng = np.random.default_rng(61 + F)
     cfg_np = rng.choice(np.array(codes, dtype=np.uint8), F)
     cfg = torch.from_numpy(cfg_np).cuda()
     md = MixedDecoder(max_iter=5, early_stop=True, codes=codes)
@@ -184,9 +185,12 @@ def test_mixed_learned_fallback_matches_per_code(gpu, oracle_mod):
     torch.cuda.synchronize()
     post, hard, ret, iters, fb = [t.cpu().numpy() for t in (post, hard, ret, iters, fb)]
     llr_np = llr.cpu().numpy().reshape(F, 2304)
-    assert 0 < fb.sum() < F
+    if F >= 1000:
+        assert 0 < fb.sum() < F
     for r in rates:
         sel = np.flatnonzero(cfg_np == code_id(r, 96))
+        if len(sel) == 0:
+            continue
         dec = W.ldpc_decoder(r, 96, 5, early_stop=True)
         dec.set_offsets(offs[r])
         d_llr = torch.from_numpy(np.ascontiguousarray(llr_np[sel])).cuda()
