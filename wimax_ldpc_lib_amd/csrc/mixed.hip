@@ -5,10 +5,12 @@
 // One decode is a hipGraph, captured once per set of buffers and replayed:
 //   prep      zero the per-code counters and persistent tile counters
 //   level     (early stop) a reliability level per frame from the mean |LLR|
-//             of its first 256 LLRs, and a counting sort of the frame ids by
-//             level (hist / scan / scatter below, over 256 levels)
-//   hist      frames per code (workgroup histogram in LDS, then one atomic per
-//             code per workgroup); invalid ids counted separately
+//             of its first 256 LLRs, the frames per level and per code in the
+//             same pass, and a counting sort of the frame ids by level (scan /
+//             scatter below, over 256 levels)
+//   hist      (no level pass) frames per code (workgroup histogram in LDS,
+//             then one atomic per code per workgroup); invalid ids counted
+//             separately
 //   scan      bucket start of each code (exclusive scan, one workgroup)
 //   scatter   frame ids into their code's bucket, in level order: a bucket is
 //             a sequence of short runs of frames of similar reliability, so the
@@ -63,16 +65,26 @@ __global__ void mixed_prep_kernel(uint32_t* counts, uint32_t* cursors, uint32_t*
 // as float bits >> 19 (exponent and 4 mantissa bits: 16 steps per octave),
 // over 2^-8 .. 2^8; lower means, zero and NaN give 0, higher ones 255.  One
 // wave per frame.  Only the order of frames within a code bucket depends on it.
+// The same pass counts the frames per level (lv_counts) and per code (counts;
+// codes >= kCodes in bin kCodes), as bucket_hist_kernel would over the level
+// and code bytes: workgroup histograms in LDS, one atomic per non-empty bin.
 __global__ __launch_bounds__(256) void mixed_level_kernel(const uint8_t* __restrict__ cfg,
                                                           const unsigned long long* __restrict__ off,
                                                           const float* __restrict__ llr, unsigned long long n,
-                                                          uint8_t* __restrict__ level)
+                                                          uint8_t* __restrict__ level, uint32_t* __restrict__ lv_counts,
+                                                          uint32_t* __restrict__ counts)
 {
+    __shared__ uint32_t hl[kLevels];
+    __shared__ uint32_t hc[kCodes + 1];
+    for (int i = threadIdx.x; i < kLevels; i += blockDim.x) hl[i] = 0;
+    for (int i = threadIdx.x; i <= kCodes; i += blockDim.x) hc[i] = 0;
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     for (unsigned long long g = (unsigned long long)blockIdx.x * 4 + (threadIdx.x >> 6); g < n;
          g += (unsigned long long)gridDim.x * 4) {
+        const int c = cfg[g];
         float sum = 0.0f;
-        if (cfg[g] < kCodes) {
+        if (c < kCodes) {
             const float* x = llr + off[g];
 #pragma unroll
             for (int q = 0; q < kLevelSample / 64; q++) sum += fabsf(x[q * 64 + lane]);
@@ -86,8 +98,15 @@ __global__ __launch_bounds__(256) void mixed_level_kernel(const uint8_t* __restr
                 lv = k < 0 ? 0 : (k >= kLevels ? kLevels - 1 : k);
             }
             level[g] = (uint8_t)lv;
+            atomicAdd(&hl[lv], 1u);
+            atomicAdd(&hc[c < kCodes ? c : kCodes], 1u);
         }
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kLevels; i += blockDim.x)
+        if (hl[i]) atomicAdd(&lv_counts[i], hl[i]);
+    for (int i = threadIdx.x; i <= kCodes; i += blockDim.x)
+        if (hc[i]) atomicAdd(&counts[i], hc[i]);
 }
 
 // Counting sort by a byte key with K buckets (keys >= K: counted in bin K,
@@ -462,14 +481,16 @@ static int capture_decode(wl_mixed* m, int learned, int fallback_iters, const ui
     const bool ordered = (learned || (m->flags & WL_FLAG_EARLY_STOP)) && !(m->flags & WL_FLAG_MIXED_UNORDERED);
     if (ordered) {
         const unsigned lblocks = (unsigned)std::min<unsigned long long>((n + 3) / 4, (unsigned long long)m->num_cu * 16);
+        // levels and both histograms in one pass
         hipLaunchKernelGGL(mixed_level_kernel, dim3(lblocks), dim3(256), 0, s, cfg,
-                           reinterpret_cast<const unsigned long long*>(frame_off), llr, n, m->level);
-        hipLaunchKernelGGL(bucket_hist_kernel<kLevels>, dim3(blocks), dim3(256), 0, s, m->level, n, m->lv_counts);
+                           reinterpret_cast<const unsigned long long*>(frame_off), llr, n, m->level, m->lv_counts,
+                           m->counts);
         hipLaunchKernelGGL(bucket_scan_kernel<kLevels>, dim3(1), dim3(64), 0, s, m->lv_counts, m->lv_buckets);
         hipLaunchKernelGGL(bucket_scatter_kernel<kLevels>, dim3(sblocks), dim3(256), 0, s, m->level,
                            (const uint32_t*)nullptr, n, m->lv_buckets, m->lv_cursors, m->lv_idx);
+    } else {
+        hipLaunchKernelGGL(bucket_hist_kernel<kCodes>, dim3(blocks), dim3(256), 0, s, cfg, n, m->counts);
     }
-    hipLaunchKernelGGL(bucket_hist_kernel<kCodes>, dim3(blocks), dim3(256), 0, s, cfg, n, m->counts);
     hipLaunchKernelGGL(bucket_scan_kernel<kCodes>, dim3(1), dim3(64), 0, s, m->counts, m->buckets);
     hipLaunchKernelGGL(bucket_scatter_kernel<kCodes>, dim3(sblocks), dim3(256), 0, s, cfg,
                        ordered ? (const uint32_t*)m->lv_idx : (const uint32_t*)nullptr, n, m->buckets, m->cursors,
