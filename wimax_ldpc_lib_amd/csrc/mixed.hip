@@ -80,7 +80,9 @@ __global__ __launch_bounds__(256) void mixed_level_kernel(const uint8_t* __restr
     for (int i = threadIdx.x; i <= kCodes; i += blockDim.x) hc[i] = 0;
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    for (unsigned long long g = (unsigned long long)blockIdx.x * 4 + (threadIdx.x >> 6); g < n;
+    // the wave's frame index is uniform: scalar loads of cfg and off
+    const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (unsigned long long g = (unsigned long long)blockIdx.x * 4 + wave; g < n;
          g += (unsigned long long)gridDim.x * 4) {
         const int c = cfg[g];
         float sum = 0.0f;
