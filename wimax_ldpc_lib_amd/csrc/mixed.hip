@@ -68,7 +68,9 @@ __global__ void mixed_prep_kernel(uint32_t* counts, uint32_t* cursors, uint32_t*
 // The same pass counts the frames per level (lv_counts) and per code (counts;
 // codes >= kCodes in bin kCodes), as bucket_hist_kernel would over the level
 // and code bytes: workgroup histograms in LDS, one atomic per non-empty bin.
-__global__ __launch_bounds__(256) void mixed_level_kernel(const uint8_t* __restrict__ cfg,
+constexpr int kLevelWaves = 16;   // waves per workgroup of the reliability pass (fewer histogram flushes)
+
+__global__ __launch_bounds__(64 * kLevelWaves) void mixed_level_kernel(const uint8_t* __restrict__ cfg,
                                                           const unsigned long long* __restrict__ off,
                                                           const float* __restrict__ llr, unsigned long long n,
                                                           uint8_t* __restrict__ level, uint32_t* __restrict__ lv_counts,
@@ -82,8 +84,8 @@ __global__ __launch_bounds__(256) void mixed_level_kernel(const uint8_t* __restr
     const int lane = threadIdx.x & 63;
     // the wave's frame index is uniform: scalar loads of cfg and off
     const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (unsigned long long g = (unsigned long long)blockIdx.x * 4 + wave; g < n;
-         g += (unsigned long long)gridDim.x * 4) {
+    for (unsigned long long g = (unsigned long long)blockIdx.x * kLevelWaves + wave; g < n;
+         g += (unsigned long long)gridDim.x * kLevelWaves) {
         const int c = cfg[g];
         float sum = 0.0f;
         if (c < kCodes) {
@@ -482,9 +484,10 @@ static int capture_decode(wl_mixed* m, int learned, int fallback_iters, const ui
     // every frame costs the same).
     const bool ordered = (learned || (m->flags & WL_FLAG_EARLY_STOP)) && !(m->flags & WL_FLAG_MIXED_UNORDERED);
     if (ordered) {
-        const unsigned lblocks = (unsigned)std::min<unsigned long long>((n + 3) / 4, (unsigned long long)m->num_cu * 16);
+        const unsigned lblocks = (unsigned)std::min<unsigned long long>((n + kLevelWaves - 1) / kLevelWaves,
+                                                                        (unsigned long long)m->num_cu * 4);
         // levels and both histograms in one pass
-        hipLaunchKernelGGL(mixed_level_kernel, dim3(lblocks), dim3(256), 0, s, cfg,
+        hipLaunchKernelGGL(mixed_level_kernel, dim3(lblocks), dim3(64 * kLevelWaves), 0, s, cfg,
                            reinterpret_cast<const unsigned long long*>(frame_off), llr, n, m->level, m->lv_counts,
                            m->counts);
         hipLaunchKernelGGL(bucket_scan_kernel<kLevels>, dim3(1), dim3(64), 0, s, m->lv_counts, m->lv_buckets);
