@@ -133,17 +133,38 @@ __global__ __launch_bounds__(256) void bucket_hist_kernel(const uint8_t* __restr
         if (h[i]) atomicAdd(&counts[i], h[i]);
 }
 
-// buckets[k] = {start, count}
+// buckets[k] = {start, count}.  One wave: lane l holds keys
+// l * per .. l * per + per - 1; a shuffle scan over the lanes' sums gives
+// each lane its first start.
 template <int K>
-__global__ void bucket_scan_kernel(const uint32_t* __restrict__ counts, uint32_t* __restrict__ buckets)
+__global__ __launch_bounds__(64) void bucket_scan_kernel(const uint32_t* __restrict__ counts,
+                                                         uint32_t* __restrict__ buckets)
 {
-    if (threadIdx.x == 0) {
-        uint32_t s = 0;
-        for (int c = 0; c < K; c++) {
+    constexpr int per = (K + 63) / 64;
+    const int lane = threadIdx.x;
+    uint32_t v[per];
+    uint32_t local = 0;
+#pragma unroll
+    for (int j = 0; j < per; j++) {
+        const int c = lane * per + j;
+        v[j] = c < K ? counts[c] : 0u;
+        local += v[j];
+    }
+    uint32_t incl = local;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    uint32_t s = incl - local;
+#pragma unroll
+    for (int j = 0; j < per; j++) {
+        const int c = lane * per + j;
+        if (c < K) {
             buckets[2 * c] = s;
-            buckets[2 * c + 1] = counts[c];
-            s += counts[c];
+            buckets[2 * c + 1] = v[j];
         }
+        s += v[j];
     }
 }
 
