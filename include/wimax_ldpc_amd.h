@@ -202,6 +202,13 @@ int wl_mixed_decode_learned_fallback_dev(wl_mixed* m, const uint8_t* cfg, const 
 /* Frames skipped by the last decode for an invalid or disabled code id
  * (synchronizes the device). */
 int wl_mixed_invalid_count(wl_mixed* m, uint64_t* out);
+/* The code buckets the last decode of the handle used (no counterpart in the
+ * reference; for checking the bucketing): buckets_host [2 * 114] gets
+ * {first slot, frame count} per code id, and idx_dev (device, nullable)
+ * [nframes] the frame ids in slot order, i.e. each code's frames from its
+ * first slot on.  nframes must not exceed the last decode's batch.
+ * Synchronizes the device. */
+int wl_mixed_bucket_list(wl_mixed* m, uint32_t* buckets_host, uint32_t* idx_dev, size_t nframes);
 /* Seeded synthetic mixed frames: frame g of code cfg[g] is exactly frame g of
  * wl_channel_dev on an encoder of that code (same seed). */
 int wl_mixed_channel_dev(wl_mixed* m, uint64_t seed, const uint8_t* cfg, const uint64_t* frame_off,

@@ -257,3 +257,27 @@ def test_quarters_code_shape(gpu, oracle_mod, rate, z):
         assert dec.variant == 1
     with pytest.raises(W.WLError):
         W.ldpc_decoder(5, 96, 5).variant = 3      # more than four waves: no such shape
+
+
+def test_nan_input_stays_in_its_frame(gpu, oracle_mod):
+    """A NaN LLR is outside the bit-exact contract: the reference is built
+    with -ffast-math (lib/CMakeLists.txt:6), under which its comparisons with
+    NaN are not defined, and the kernel's min3 / med3 selection does not skip
+    NaN as the oracle's strict '<' does.  What is guaranteed: the decode of a
+    batch with a NaN frame completes, the NaN posterior stays NaN, and every
+    other frame of the batch (including the frames sharing the NaN frame's
+    workgroup) equals the oracle bit for bit."""
+    import wimax_ldpc_lib_amd as W
+    for rate, z in ((5, 96), (0, 24), (3, 68)):
+        code = oracle_mod.Code(rate, z)
+        llr, _ = noisy_llr(code, 9, 2.5, seed=rate * 100 + z)
+        bad = 4
+        llr[bad, 17] = np.float32("nan")
+        keep = np.array([f for f in range(9) if f != bad])
+        for early in (True, False):
+            o = code.decode(llr[keep], 10, early)
+            dec = W.ldpc_decoder(rate, z, 10, early_stop=early)
+            g = _gpu_decode(gpu, dec, llr)
+            assert np.isnan(g["post"][bad, 17])
+            gk = {k: v[keep] for k, v in g.items()}
+            _compare(gk, o, "NaN batch rate %d z %d early %d" % (rate, z, early))

@@ -156,8 +156,7 @@ def test_mixed_reliability_order_leaves_results_unchanged(gpu):
     """Frames arranged by reliability inside the code buckets (the early-stop
     default) decode exactly as without the arrangement, including frames whose
     reliability estimate falls off either end of its range (all-zero LLRs,
-    tiny and huge magnitudes, a NaN), and those frames equal a per-code
-    decode."""
+    tiny and huge magnitudes), and those frames equal a per-code decode."""
     import wimax_ldpc_lib_amd as W
     from wimax_ldpc_lib_amd.mixed import MixedDecoder
     torch = gpu
@@ -190,3 +189,90 @@ def test_mixed_reliability_order_leaves_results_unchanged(gpu):
         assert np.array_equal(post[o:o + n].view(np.uint32), p1.cpu().numpy()[0].view(np.uint32)), g
         assert np.array_equal(hard[o // 8:(o + n) // 8], h1.cpu().numpy()[0]), g
         assert ret[g] == r1.cpu().numpy()[0] and iters[g] == i1.cpu().numpy()[0], g
+
+
+def _gather_index(torch, offs, sel, n):
+    """[len(sel), n] element indices of frames sel of a ragged buffer (device)."""
+    base = torch.from_numpy(offs[sel].astype(np.int64)).cuda()
+    return base[:, None] + torch.arange(n, device="cuda", dtype=torch.int64)[None, :]
+
+
+def test_mixed_full_size_262144_frames(gpu):
+    """The mixed workload at its full size: 262144 frames, rate uniform over
+    the five encoder rates and z over the 19 lifting sizes (95 codes), Eb/N0
+    uniform in [1, 4.5] dB per frame, max 10 iterations, early stop; one
+    captured graph.  Checked: every frame is in exactly one code bucket, its
+    own; every frame equals the per-code decoder (posteriors, hard decisions,
+    return value, iterations, syndrome weight); a second replay into
+    sentinel-filled outputs is identical; a spread sample equals the oracle.
+    (A group kernel that kept its tile table in LDS decoded a few frames of
+    each such batch from the next code's bucket at random; batches of a few
+    thousand frames, one tile per workgroup, did not show it.)"""
+    import wimax_ldpc_lib_amd as W
+    from oracle import oracle as O
+    torch = gpu
+    F = 262144
+    codes = [r * 19 + zi for r in (0, 1, 2, 3, 5) for zi in range(19)]
+    md, cfg_np, cfg, off, offs, n_of, llr, bits = _setup(torch, F, 31337, codes)
+    total = llr.numel()
+
+    def run():
+        post = torch.full((total,), float("nan"), dtype=torch.float32, device="cuda")
+        hard = torch.full((total // 8,), 0xAB, dtype=torch.uint8, device="cuda")
+        ret = torch.full((F,), 0xAB, dtype=torch.uint8, device="cuda")
+        iters = torch.full((F,), 0xAB, dtype=torch.uint8, device="cuda")
+        synd = torch.full((F,), -1, dtype=torch.int16, device="cuda")
+        torch.cuda.synchronize()
+        md.decode_dev(cfg, off, llr, post, hard, ret, iters, synd)
+        torch.cuda.synchronize()
+        return post, hard, ret, iters, synd
+
+    first = run()
+    # (1) buckets: the bucket of code c holds exactly the frames of code c
+    idx = torch.empty(F, dtype=torch.int32, device="cuda")
+    b = md.bucket_list(idx).astype(np.int64)
+    idx_np = idx.cpu().numpy().astype(np.int64)
+    assert b[:, 1].sum() == F and len(np.unique(idx_np)) == F
+    for c in range(114):
+        got = np.sort(idx_np[b[c, 0]:b[c, 0] + b[c, 1]])
+        assert np.array_equal(got, np.flatnonzero(cfg_np == c)), c
+    # (2) every frame against the per-code decoder, on the device
+    post, hard, ret, iters, synd = first
+    for c in codes:
+        rate, z = c // 19, 24 + 4 * (c % 19)
+        n = 24 * z
+        sel = np.flatnonzero(cfg_np == c)
+        gi = _gather_index(torch, offs, sel, n)
+        hi = _gather_index(torch, offs // 8, sel, n // 8)
+        dec = W.ldpc_decoder(rate, z, 10, early_stop=True)
+        frames = llr[gi].contiguous()
+        p1 = torch.empty_like(frames)
+        h1 = torch.empty((len(sel), n // 8), dtype=torch.uint8, device="cuda")
+        r1 = torch.empty(len(sel), dtype=torch.uint8, device="cuda")
+        i1 = torch.empty(len(sel), dtype=torch.uint8, device="cuda")
+        s1 = torch.empty(len(sel), dtype=torch.int16, device="cuda")
+        dec.decode_dev(frames, p1, h1, r1, i1, s1)
+        torch.cuda.synchronize()
+        s = torch.from_numpy(sel).cuda()
+        assert torch.equal(post[gi].view(torch.int32), p1.view(torch.int32)), c
+        assert torch.equal(hard[hi], h1), c
+        assert torch.equal(ret[s], r1) and torch.equal(iters[s], i1) and torch.equal(synd[s], s1), c
+    # (3) a second replay, outputs re-filled with sentinels
+    second = run()
+    for a, b2 in zip(first, second):
+        assert torch.equal(a.view(torch.uint8), b2.view(torch.uint8))
+    # (4) a spread sample against the oracle: two frames of every code
+    llr_np = llr.cpu().numpy()
+    post_np, hard_np = post.cpu().numpy(), hard.cpu().numpy()
+    ret_np, it_np, sy_np = ret.cpu().numpy(), iters.cpu().numpy(), synd.cpu().numpy()
+    rng = np.random.default_rng(7)
+    for c in codes:
+        rate, z = c // 19, 24 + 4 * (c % 19)
+        n = 24 * z
+        sel = rng.choice(np.flatnonzero(cfg_np == c), 2, replace=False)
+        o = O.Code(rate, z).decode(np.stack([llr_np[offs[g]:offs[g] + n] for g in sel]), 10, True)
+        for q, g in enumerate(sel):
+            assert bits_equal_f32(post_np[offs[g]:offs[g] + n], o["post"][q]), (c, g)
+            assert np.array_equal(hard_np[offs[g] // 8:(offs[g] + n) // 8],
+                                  np.packbits(o["hd"][q].astype(np.uint8), bitorder="little")), (c, g)
+            assert ret_np[g] == o["ret"][q] and it_np[g] == o["iters"][q] and sy_np[g] == o["synd_w"][q], (c, g)

@@ -46,6 +46,7 @@ SIGNATURES = {
     "wl_mixed_set_offsets": (C.c_int, [_P, C.c_int, _P, C.c_int]),
     "wl_mixed_decode_learned_fallback_dev": (C.c_int, [_P, _P, _P, _P, _SZ, C.c_int, _P, _P, _P, _P, _P, _P, _P]),
     "wl_mixed_invalid_count": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "wl_mixed_bucket_list": (C.c_int, [_P, _P, _P, _SZ]),
     "wl_mixed_channel_dev": (C.c_int, [_P, C.c_uint64, _P, _P, _SZ, C.c_float, C.c_float, _P, _P, _P, _P]),
     "wl_trainer_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_float, C.c_float,
                                      C.POINTER(_P)]),

@@ -65,12 +65,6 @@ struct GroupShape {
     }
 };
 
-// LDS pad offset of the group's tile table: kNumRates + 1 u64 (tiles before
-// rate r, then the total), then kNumRates u64 (frames of rate r).  The
-// learned fallback queue uses the same bytes in kernels that never run a
-// group.
-constexpr int kGroupTilesOff = 128;
-
 template <int Z>
 __global__ __launch_bounds__(kMaxThreads) void tdmp_group_kernel(GroupArgs g)
 {
@@ -78,47 +72,36 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_group_kernel(GroupArgs g)
     using S = GroupShape<Z>;
     constexpr int F = S::F;
     constexpr int zi = index_of_z(Z);
-    static_assert(kGroupTilesOff + 8 * (2 * kNumRates + 1) <= kTileSlot, "tile table overlaps the tile slot");
     unsigned long long* slot = reinterpret_cast<unsigned long long*>(smem + kTileSlot);
-    unsigned long long* tiles = reinterpret_cast<unsigned long long*>(smem + kGroupTilesOff);
-    unsigned long long* counts = tiles + kNumRates + 1;
     // Tiles of rate 0 first, then rate 1, ...: the counter only grows, so
-    // every workgroup reaches the exit once the last tile is taken.  The
-    // bucket counts are final when the kernel starts: the tile boundaries per
-    // rate are read once into LDS.  Each tile index is claimed one tile
-    // ahead, so the atomic's latency overlaps the current tile.
-    if (threadIdx.x == 0) {
-        unsigned long long c[kNumRates];
-#pragma unroll
-        for (int r = 0; r < kNumRates; r++)
-            c[r] = ((g.enabled >> r) & 1u) ? g.buckets[2 * (r * kNumZ + zi) + 1] : 0ull;
-        unsigned long long acc = 0;
-#pragma unroll
-        for (int r = 0; r < kNumRates; r++) {
-            tiles[r] = acc;
-            counts[r] = c[r];
-            acc += (c[r] + F - 1) / F;
-        }
-        tiles[kNumRates] = acc;
-        *slot = atomicAdd(g.tile_counter, 1ull);
-    }
-    // values read from LDS that are the same in every thread, made scalar
-    auto uniform = [](unsigned long long v) -> unsigned long long {
-        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-        return ((unsigned long long)hi << 32) | lo;
-    };
+    // every workgroup reaches the exit once the last tile is taken.  Nothing
+    // is kept in LDS from one tile to the next: the tile index is written and
+    // read between two barriers, and the tile counts per rate are re-read
+    // from the device bucket table for every tile.  (A form that kept the
+    // per-rate tile boundaries in the LDS pad for the whole kernel and
+    // claimed each tile index one tile ahead decoded a few frames per
+    // 262144-frame batch from the next code's bucket, at random; see the
+    // full-size mixed test.)
     for (;;) {
+        if (threadIdx.x == 0) *slot = atomicAdd(g.tile_counter, 1ull);
         __syncthreads();
-        const unsigned long long t = uniform(*slot);
+        unsigned long long t = *slot;
         __syncthreads();
-        if (t >= uniform(tiles[kNumRates])) break;
-        unsigned long long next = 0;
-        if (threadIdx.x == 0) next = atomicAdd(g.tile_counter, 1ull);
-        int rate = 0;
-        while (t >= uniform(tiles[rate + 1])) rate++;
-        const unsigned long long frame0 = (t - uniform(tiles[rate])) * F;
-        const unsigned long long left = uniform(counts[rate]) - frame0;
+        int rate = -1;
+        unsigned long long cnt = 0;
+        for (int r = 0; r < kNumRates; r++) {
+            const unsigned long long c = ((g.enabled >> r) & 1u) ? g.buckets[2 * (r * kNumZ + zi) + 1] : 0ull;
+            const unsigned long long nt = (c + F - 1) / F;
+            if (t < nt) {
+                rate = r;
+                cnt = c;
+                break;
+            }
+            t -= nt;
+        }
+        if (rate < 0) break;   // uniform: t and the counts are the same in every thread
+        const unsigned long long frame0 = t * F;
+        const unsigned long long left = cnt - frame0;
         const int fe = left < (unsigned long long)F ? (int)left : F;
         static_for<0, kNumRates>([&](auto Rc) {
             constexpr int R = decltype(Rc)::value;
@@ -128,7 +111,7 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_group_kernel(GroupArgs g)
             a.frames_per_wg = F;
             tdmp_tile<R, Z, MODE_NMS, S::template var<R>(), true>(a, smem, frame0, fe);
         });
-        if (threadIdx.x == 0) *slot = next;
+        __syncthreads();
     }
 }
 

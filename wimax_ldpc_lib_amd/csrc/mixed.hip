@@ -688,6 +688,19 @@ int wl_mixed_decode_learned_fallback_dev(wl_mixed* m, const uint8_t* cfg, const 
                      (hipStream_t)stream);
 }
 
+int wl_mixed_bucket_list(wl_mixed* m, uint32_t* buckets_host, uint32_t* idx_dev, size_t nframes)
+{
+    if (!m || !buckets_host) return mfail(WL_EINVAL, "null argument");
+    if (idx_dev && nframes > m->idx_cap) return mfail(WL_EINVAL, "nframes exceeds the last decode's batch");
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess)
+        e = hipMemcpy(buckets_host, m->buckets, 2 * kCodes * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && idx_dev && nframes)
+        e = hipMemcpy(idx_dev, m->idx, nframes * sizeof(uint32_t), hipMemcpyDeviceToDevice);
+    if (e != hipSuccess) return hipf(e, "read bucket list");
+    return WL_OK;
+}
+
 int wl_mixed_invalid_count(wl_mixed* m, uint64_t* out)
 {
     if (!m || !out) return mfail(WL_EINVAL, "null argument");

@@ -68,6 +68,15 @@ class MixedDecoder:
                                                          _ptr(iters), _ptr(synd), _ptr(fallback),
                                                          _stream(stream)))
 
+    def bucket_list(self, idx=None):
+        """{first slot, count} per code id ([114, 2] numpy) of the last
+        decode; with idx (a uint32/int32 device tensor of the batch size) also
+        the frame ids in slot order."""
+        import numpy as np
+        b = np.zeros((114, 2), dtype=np.uint32)
+        check(lib().wl_mixed_bucket_list(self._h, _ptr(b), _ptr(idx), 0 if idx is None else idx.numel()))
+        return b
+
     def invalid_count(self):
         v = C.c_uint64()
         check(lib().wl_mixed_invalid_count(self._h, C.byref(v)))
