@@ -51,8 +51,12 @@ namespace wimax_ldpc_lib {
             void fill_with_random(uint8_t* buffer, unsigned int buf_len);
             unsigned int compare_data(const uint8_t* original_data, const int8_t* decoded_data);
             void make_frame(double EbNo_dB, uint8_t* dataword, uint8_t* codeword, float* channel);
+            void require_code() const;
 
         public:
+            /* include/BER_runner.h:51: no code; run_iteration / run_batch /
+             * run_until_error throw std::exception on such an object */
+            BER_runner();
             BER_runner(coderate rate, unsigned int z_factor,
                        unsigned int num_threads,
                        unsigned int max_iter);

@@ -35,6 +35,12 @@ def test_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(L, name), name
     assert declared == set(capi.SIGNATURES), declared ^ set(capi.SIGNATURES)
+    # the data symbols of matrices.h: 114 tables and their D / X constants
+    data = re.findall(r"^extern\s+[\w\s]+?\b(wimax_\w+)", open(os.path.join(ROOT, "include", "matrices.h")).read(),
+                      flags=re.M)
+    assert len(data) == 3 * 114
+    for name in data:
+        assert hasattr(L, name), name
     assert W.lib().wl_version().startswith(b"wimax_ldpc_amd")
 
 
@@ -47,6 +53,32 @@ def test_code_params_match_reference_hashes():
         assert (p["n"], p["m"], p["c"], p["d"], p["x"]) == (r["n"], r["m"], r["c"], r["d"], r["x"])
         t = W.ldpc(rate, z).checknode_array()
         assert hashlib.sha256(np.ascontiguousarray(t, dtype="<i2").tobytes()).hexdigest() == r["sha256"]
+
+
+def test_matrices_header_tables_equal_reference(tmp_path):
+    """include/matrices.h: a C++ caller that names the tables (wimax_2304_0_83,
+    ...) and reads them through ldpc::m_checknode_array gets the reference's
+    expanded tables and D / X for all 114 codes (include/matrices.h:1-824,
+    lib/matrices/*.cc), and BER_runner() (include/BER_runner.h:51) builds."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(GOLDEN), "cpp", "_bin", "matrices_check")
+    assert os.path.exists(exe), "built by wimax_ldpc_lib_amd/build.py"
+    out = tmp_path / "tables.bin"
+    p = subprocess.run([exe, str(out)], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 0, p.stdout + p.stderr
+    ref = json.load(open(os.path.join(GOLDEN, "tables_ref.json")))
+    buf = out.read_bytes()
+    pos, seen = 0, 0
+    while pos < len(buf):
+        rate, z, rows, cols, d, x = np.frombuffer(buf, dtype="<i4", count=6, offset=pos)
+        pos += 24
+        t = np.frombuffer(buf, dtype="<i2", count=rows * cols, offset=pos)
+        pos += 2 * rows * cols
+        r = ref["%d_%d" % (rate, z)]
+        assert (rows, cols, d, x) == (r["m"], r["c"], r["d"], r["x"]), (rate, z)
+        assert hashlib.sha256(t.tobytes()).hexdigest() == r["sha256"], (rate, z)
+        seen += 1
+    assert seen == 114
 
 
 @pytest.mark.parametrize("rate,z", [(6, 96), (-1, 96), (0, 20), (0, 98), (0, 50), (5, 0)])

@@ -38,7 +38,7 @@ HIP_SOURCES = ["decode_r0.hip", "decode_r1.hip", "decode_r2.hip", "decode_r3.hip
                "decode_r5.hip", "encoder.hip", "capi.hip", "learned.hip", "trainer.hip", "mixed.hip",
                "generic.hip", "qc_host.hip", "decode_g0.hip", "decode_g1.hip", "decode_g2.hip", "decode_g3.hip",
                "decode_g4.hip"]
-CXX_SOURCES = ["ldpc_classes.cpp"]
+CXX_SOURCES = ["ldpc_classes.cpp", "matrices.cpp"]
 BINDIR = os.path.join(PKG, "bin")
 # host programs linked against the library: (source, output, rpath relative to the output)
 PROGRAMS = [
@@ -47,6 +47,8 @@ PROGRAMS = [
     (os.path.join(PKG, "cli", "BER_example.cpp"), os.path.join(BINDIR, "BER_example"), "$ORIGIN/../lib"),
     (os.path.join(PKG, "cli", "Capacity_example.cpp"), os.path.join(BINDIR, "Capacity_example"), "$ORIGIN/../lib"),
     (os.path.join(ROOT, "tests", "cpp", "drop_in_check.cpp"), os.path.join(ROOT, "tests", "cpp", "_bin", "drop_in_check"),
+     "$ORIGIN/../../../wimax_ldpc_lib_amd/lib"),
+    (os.path.join(ROOT, "tests", "cpp", "matrices_check.cpp"), os.path.join(ROOT, "tests", "cpp", "_bin", "matrices_check"),
      "$ORIGIN/../../../wimax_ldpc_lib_amd/lib"),
 ]
 

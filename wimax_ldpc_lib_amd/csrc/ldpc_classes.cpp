@@ -13,6 +13,7 @@
 #include "ldpc.h"
 #include "ldpc_decoder.h"
 #include "ldpc_encoder.h"
+#include "matrices.hpp"
 #include "wimax_ldpc_amd.h"
 
 namespace wimax_ldpc_lib {
@@ -48,9 +49,9 @@ ldpc::ldpc(coderate rate, unsigned int z_factor, unsigned int num_threads) : ldp
     m_X = ci.x;
     m_row_size = ci.m;
     m_col_size = ci.c;
-    m_table.resize((size_t)ci.m * ci.c);
-    check(wl_code_table((int)rate, (int)z_factor, m_table.data(), m_table.size()), "wl_code_table");
-    m_checknode_array = m_table.data();
+    // the code's table of matrices.h, as set_checknode_array (lib/ldpc.cc:69-1206)
+    m_checknode_array = wl::matrix_table((int)rate, (int)z_factor);
+    if (!m_checknode_array) throw std::runtime_error("no table for this code");
 }
 
 ldpc::~ldpc() {}
@@ -161,6 +162,23 @@ void ldpc_encoder::encode_batch(const uint8_t* infoword, size_t nframes, uint8_t
 
 #define BITS_PER_SYM 1   // lib/BER_runner.cc:9
 
+// lib/BER_runner.cc:13: a runner with no code.  The reference leaves every
+// member unset; here the counters are zero and the frame methods throw.
+BER_runner::BER_runner()
+    : ldpc(), m_encoder(nullptr), m_decoder(nullptr), m_num_bits(0), m_num_errors(0), m_codeword_len(0),
+      m_dataword_len(0), m_float_rate(0.0f), m_lfsr(0xdeadbeef), m_lfsr_start_state(0xdeadbeef), m_lfsr_index(0),
+      m_lfsr_reset_len(1152 * 100)
+{
+}
+
+void BER_runner::require_code() const
+{
+    if (!m_encoder || !m_decoder) {
+        printf("[!]ber_runner - no code: constructed with BER_runner()\n");
+        throw std::exception();
+    }
+}
+
 // lib/BER_runner.cc:15-87
 BER_runner::BER_runner(coderate rate, unsigned int z_factor, unsigned int num_threads, unsigned int max_iter)
     : ldpc(rate, z_factor, num_threads), m_encoder(nullptr), m_decoder(nullptr)
@@ -263,6 +281,7 @@ void BER_runner::make_frame(double EbNo_dB, uint8_t* dataword, uint8_t* codeword
 // lib/BER_runner.cc:162-204
 double BER_runner::run_iteration(double EbNo_dB)
 {
+    require_code();
     std::vector<uint8_t> data(m_dataword_len), cw(m_codeword_len);
     std::vector<float> ch(m_codeword_len);
     std::vector<int8_t> dec(m_codeword_len);
@@ -277,6 +296,7 @@ double BER_runner::run_iteration(double EbNo_dB)
 
 double BER_runner::run_batch(double EbNo_dB, size_t nframes)
 {
+    require_code();
     if (nframes == 0) return m_num_bits ? (double)m_num_errors / (double)m_num_bits : 0.0;
     const size_t k = m_dataword_len, n = m_codeword_len;
     std::vector<uint8_t> data(nframes * k), cw(nframes * n);
@@ -295,6 +315,7 @@ double BER_runner::run_batch(double EbNo_dB, size_t nframes)
 
 bool BER_runner::run_until_error(double EbNo_dB, size_t max_frames, size_t* frames_run)
 {
+    require_code();
     const size_t k = m_dataword_len, n = m_codeword_len;
     size_t done = 0, chunk = 32;
     while (done < max_frames) {
