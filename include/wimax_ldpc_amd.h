@@ -146,7 +146,9 @@ int wl_decoder_set_offsets(wl_decoder* dec, const float* offsets_host, int T);
 /* Learned decode with fallback: frames whose hard decisions fail the
  * syndrome after the learned iterations are decoded again from their input
  * LLRs with normalised min-sum, fallback_iters iterations, early stop.
- * fallback (nullable) [nframes] gets 1 for re-decoded frames.  For z = 24 and
+ * fallback (nullable) [nframes] gets 1 for re-decoded frames, 0 for the
+ * others.  post must not overlap llr (WL_EINVAL): the failed frames are
+ * decoded again from their input LLRs.  For z = 24 and
  * z = 96 this is one persistent kernel that re-queues failed frames in the
  * workgroup and decodes them itself; other z use a learned pass, a device-side
  * compaction of the failures and a normalised min-sum pass over them. */
@@ -193,7 +195,8 @@ int wl_mixed_set_offsets(wl_mixed* m, int rate, const float* offsets_host, int T
  * learned decoder (max_iter iterations, the early-stop flag of create()) and
  * re-queues each frame whose syndrome still fails to normalised min-sum with
  * fallback_iters iterations and early stop, from the frame's input LLRs, in
- * the same kernel; fallback [nframes] (nullable) gets 1 for such frames.
+ * the same kernel; fallback [nframes] (nullable) gets 1 for such frames
+ * and 0 for every other frame.  post may not be llr (WL_EINVAL).
  * Other codes are decoded with normalised min-sum as wl_mixed_decode_dev. */
 int wl_mixed_decode_learned_fallback_dev(wl_mixed* m, const uint8_t* cfg, const uint64_t* frame_off,
                                          const float* llr, size_t nframes, int fallback_iters, float* post,

@@ -40,7 +40,7 @@ def test_learned_inference_matches_oracle(gpu, oracle_mod, rate, z):
         assert np.array_equal(g["synd"], o["synd_w"])
 
 
-@pytest.mark.parametrize("rate,z", [(1, 24), (0, 24), (5, 96)])
+@pytest.mark.parametrize("rate,z", [(1, 24), (0, 24), (5, 96), (1, 96)])
 def test_gradients_match_oracle(gpu, oracle_mod, rate, z):
     """GPU forward + backward against the oracle's exact backward: same loss
     and final posteriors, gradients within 1e-3 relative (float summation
@@ -82,6 +82,11 @@ def test_training_reduces_loss_and_resumes(gpu, oracle_mod, tmp_path):
     assert tr2.global_step() == 301 and np.array_equal(tr2.offsets(), a)
     lb = tr2.step(20, 1.0, 4.0)
     assert la == lb and np.array_equal(tr.offsets(), tr2.offsets())   # resumed run is identical
+    # another batch size would continue another frame stream: refused
+    import wimax_ldpc_lib_amd as W
+    tr3 = Trainer(1, 24, T=5, batch=256, seed=7, lr=1e-2)
+    with pytest.raises(W.WLError, match="batch"):
+        tr3.load(ck)
 
 
 @pytest.mark.parametrize("z,order", [(96, "split"), (96, "interleaved"), (48, "split"), (24, "interleaved"),
@@ -205,9 +210,90 @@ def test_mixed_learned_fallback_matches_per_code(gpu, oracle_mod, F):
         assert np.array_equal(hard.reshape(F, 288)[sel], h1.cpu().numpy())
         assert np.array_equal(ret[sel], r1.cpu().numpy()) and np.array_equal(fb[sel], f1.cpu().numpy())
         assert np.array_equal(iters[sel], i1.cpu().numpy())
-        # fallback frames: the oracle's normalised min-sum, 10 iterations
+        # fallback frames: the oracle's normalised min-sum, 10 iterations;
+        # learned-pass frames: the oracle's learned decoder, 5 iterations
         code = oracle_mod.Code(r, 96)
         fsel = sel[fb[sel] == 1][:8]
         if len(fsel):
             o = code.decode(llr_np[fsel], 10, True)
             assert bits_equal_f32(post.reshape(F, 2304)[fsel], o["post"])
+            assert np.array_equal(ret[fsel], o["ret"]) and np.array_equal(iters[fsel], o["iters"])
+        lsel = sel[fb[sel] == 0][:24]
+        if len(lsel):
+            o = code.decode(llr_np[lsel], 5, True, variant=oracle_mod.LEARNED, offsets=offs[r])
+            assert bits_equal_f32(post.reshape(F, 2304)[lsel], o["post"])
+            assert np.array_equal(hard.reshape(F, 288)[lsel], pack_bits(o["hd"]))
+            assert (ret[lsel] == 0).all() and np.array_equal(o["ret"], ret[lsel])
+            assert np.array_equal(iters[lsel], o["iters"])
+
+
+def test_mixed_learned_fallback_other_codes(gpu, oracle_mod):
+    """A learned + fallback graph over codes with offsets (rate 2/3A at z = 96)
+    and codes without (rate 3/4A at z = 96, rate 1/2 at z = 48): the latter
+    decode with normalised min-sum as in the plain graph, and their fallback
+    entries are 0 (the caller's buffer starts at 7)."""
+    import wimax_ldpc_lib_amd as W
+    from wimax_ldpc_lib_amd.mixed import MixedDecoder, code_id
+    torch = gpu
+    codes = [code_id(1, 96), code_id(3, 96), code_id(0, 48)]
+    F = 900
+    rng = np.random.default_rng(71)
+    cfg_np = rng.choice(np.array(codes, dtype=np.uint8), F)
+    cfg = torch.from_numpy(cfg_np).cuda()
+    md = MixedDecoder(max_iter=5, early_stop=True, codes=codes)
+    c1 = oracle_mod.Code(1, 96)
+    off1 = rng.uniform(0.1, 0.5, (5, c1.mb, c1.c)).astype(np.float32)
+    md.set_offsets(1, off1)
+    off = torch.empty(F, dtype=torch.int64, device="cuda")
+    md.offsets_dev(cfg, off)
+    n_of = np.array([24 * (24 + 4 * (int(c) % 19)) for c in cfg_np], dtype=np.int64)
+    offs = np.concatenate([[0], np.cumsum(n_of)[:-1]])
+    llr = torch.empty(int(n_of.sum()), dtype=torch.float32, device="cuda")
+    md.channel_dev(72, cfg, off, 1.5, 5.0, llr)
+    post = torch.empty_like(llr)
+    ret = torch.empty(F, dtype=torch.uint8, device="cuda")
+    iters = torch.empty(F, dtype=torch.uint8, device="cuda")
+    fb = torch.full((F,), 7, dtype=torch.uint8, device="cuda")
+    md.decode_learned_fallback_dev(cfg, off, llr, 10, post, None, ret, iters, None, fb)
+    torch.cuda.synchronize()
+    post, ret, iters, fb = [t.cpu().numpy() for t in (post, ret, iters, fb)]
+    llr_np = llr.cpu().numpy()
+    assert set(np.unique(fb)) <= {0, 1}
+    for c in codes:
+        rate, z = c // 19, 24 + 4 * (c % 19)
+        n = 24 * z
+        sel = np.flatnonzero(cfg_np == c)
+        frames = np.stack([llr_np[offs[g]:offs[g] + n] for g in sel])
+        if rate == 1:
+            continue   # covered against per-code learned decoding above
+        # no offsets for this rate: the plain decoder with the graph's
+        # max_iter, fallback 0
+        o = oracle_mod.Code(rate, z).decode(frames, 5, True)
+        assert (fb[sel] == 0).all(), c
+        for q, g in enumerate(sel):
+            assert bits_equal_f32(post[offs[g]:offs[g] + n], o["post"][q]), (c, g)
+        assert np.array_equal(ret[sel], o["ret"]) and np.array_equal(iters[sel], o["iters"])
+
+
+def test_learned_fallback_rejects_in_place_posteriors(gpu, oracle_mod):
+    """The failed frames are decoded again from their input LLRs, so post
+    may not overlap llr (the plain decode allows in place)."""
+    import wimax_ldpc_lib_amd as W
+    from wimax_ldpc_lib_amd.mixed import MixedDecoder, code_id
+    torch = gpu
+    code = oracle_mod.Code(1, 96)
+    llr_np, _ = noisy_llr(code, 4, 2.0, seed=81)
+    dec = W.ldpc_decoder(1, 96, 5, early_stop=True)
+    dec.set_offsets(np.full((5, code.mb, code.c), 0.15, np.float32))
+    llr = torch.from_numpy(llr_np).cuda()
+    ret = torch.empty(4, dtype=torch.uint8, device="cuda")
+    for post in (llr, llr[1:]):
+        with pytest.raises(W.WLError, match="overlaps"):
+            dec.decode_learned_fallback_dev(llr, 10, post, None, ret)
+    md = MixedDecoder(max_iter=5, early_stop=True, codes=[code_id(1, 96)])
+    cfg = torch.full((4,), code_id(1, 96), dtype=torch.uint8, device="cuda")
+    off = torch.empty(4, dtype=torch.int64, device="cuda")
+    md.offsets_dev(cfg, off)
+    flat = llr.view(-1)
+    with pytest.raises(W.WLError, match="post is llr"):
+        md.decode_learned_fallback_dev(cfg, off, flat, 10, flat, None, ret)

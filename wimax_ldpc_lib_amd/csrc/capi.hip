@@ -429,24 +429,44 @@ int wl_decode_f32_dev(wl_decoder* dec, const float* llr, size_t nframes, float* 
                          nframes, post, hard, ret, iters, synd, nullptr, nullptr, (hipStream_t)stream);
 }
 
-int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nframes,
-                                   int fallback_iters, float* post, uint8_t* hard, uint8_t* ret,
-                                   uint8_t* iters, uint16_t* synd, uint8_t* fallback, void* stream)
+}  // extern "C"
+
+namespace {
+
+// Argument checks shared by the two learned-fallback entry points.  The
+// failed frames are decoded again from their input LLRs, so the posterior
+// output may not overlap the input (in place would re-decode the learned
+// posteriors instead).
+int check_learned_fallback(const wl_decoder* dec, const float* llr, size_t nframes, int fallback_iters,
+                           const float* post, const uint8_t* ret)
 {
     if (!dec) return fail(WL_EINVAL, "null decoder");
     if (dec->T <= 0) return fail(WL_EINVAL, "no learned offsets set");
+    if (dec->generic) return fail(WL_ENOTSUP, "learned offsets need a compiled 802.16e code");
     if (nframes == 0) return WL_OK;
     if (!llr || !ret) return fail(WL_EINVAL, "llr and ret are required");
     if (fallback_iters < 0 || fallback_iters > 255) return fail(WL_EINVAL, "fallback_iters out of range");
-    if (dec->generic) return fail(WL_ENOTSUP, "learned offsets need a compiled 802.16e code");
     if (nframes > 0xffffffffull) return fail(WL_EINVAL, "nframes %zu too large", nframes);
-    hipStream_t s = (hipStream_t)stream;
-    int rc;
+    if (post) {
+        const size_t len = nframes * (size_t)dec->code.info.n;
+        if (post < llr + len && llr < post + len)
+            return fail(WL_EINVAL, "post overlaps llr: re-queued frames are decoded again from the input LLRs");
+    }
+    return WL_OK;
+}
+
+// The learned decode with fallback over caller-provided scratch: counter
+// (two uint32 = the persistent kernel's 64-bit tile counter, or the
+// fallback list header {first slot, count}) and idx [nframes] (the list of
+// failed frames, used where z is not 24 or 96).
+int learned_fallback(wl_decoder* dec, const float* llr, size_t nframes, int fallback_iters, float* post,
+                     uint8_t* hard, uint8_t* ret, uint8_t* iters, uint16_t* synd, uint8_t* fallback,
+                     uint32_t* counter, uint32_t* idx, hipStream_t s)
+{
+    hipLaunchKernelGGL(zero_bucket_kernel, dim3(1), dim3(1), 0, s, counter);
     if (dec->entry->learned_fb) {
         // one persistent kernel: learned tiles, failures re-queued in LDS and
         // decoded again with normalised min-sum by the same workgroup
-        if ((rc = dec->d_count.ensure(2))) return rc;
-        hipLaunchKernelGGL(zero_bucket_kernel, dim3(1), dim3(1), 0, s, dec->d_count.p);
         wl::DecodeArgs a{};
         a.llr = llr;
         a.post = post;
@@ -455,7 +475,7 @@ int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nfr
         a.iters = iters;
         a.synd = synd;
         a.offsets = dec->offsets.p;
-        a.tile_counter = reinterpret_cast<unsigned long long*>(dec->d_count.p);
+        a.tile_counter = reinterpret_cast<unsigned long long*>(counter);
         a.nframes = nframes;
         a.frames_per_wg = dec->frames_per_wg;
         a.max_iter = dec->max_iter;
@@ -472,18 +492,31 @@ int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nfr
         if (e != hipSuccess) return hip_fail(e, "learned fallback kernel launch");
         return WL_OK;
     }
-    rc = decode_launch(dec, true, dec->max_iter, (dec->flags & WL_FLAG_EARLY_STOP) != 0, llr, nframes, post, hard,
-                       ret, iters, synd, nullptr, nullptr, s);
+    int rc = decode_launch(dec, true, dec->max_iter, (dec->flags & WL_FLAG_EARLY_STOP) != 0, llr, nframes, post,
+                           hard, ret, iters, synd, nullptr, nullptr, s);
     if (rc) return rc;
-    if ((rc = dec->d_idx.ensure(nframes))) return rc;
-    if ((rc = dec->d_count.ensure(2))) return rc;
-    hipLaunchKernelGGL(zero_bucket_kernel, dim3(1), dim3(1), 0, s, dec->d_count.p);
     const unsigned blocks = (unsigned)((nframes + 255) / 256);
-    hipLaunchKernelGGL(compact_failed_kernel, dim3(blocks), dim3(256), 0, s, ret,
-                       (unsigned long long)nframes, dec->d_idx.p, dec->d_count.p + 1, fallback);
+    hipLaunchKernelGGL(compact_failed_kernel, dim3(blocks), dim3(256), 0, s, ret, (unsigned long long)nframes, idx,
+                       counter + 1, fallback);
     WL_HIP(hipGetLastError());
-    return decode_launch(dec, false, fallback_iters, true, llr, nframes, post, hard, ret, iters, synd,
-                         dec->d_idx.p, dec->d_count.p, s);
+    return decode_launch(dec, false, fallback_iters, true, llr, nframes, post, hard, ret, iters, synd, idx, counter,
+                         s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nframes,
+                                   int fallback_iters, float* post, uint8_t* hard, uint8_t* ret,
+                                   uint8_t* iters, uint16_t* synd, uint8_t* fallback, void* stream)
+{
+    int rc = check_learned_fallback(dec, llr, nframes, fallback_iters, post, ret);
+    if (rc || nframes == 0) return rc;
+    if ((rc = dec->d_count.ensure(2))) return rc;
+    if (!dec->entry->learned_fb && (rc = dec->d_idx.ensure(nframes))) return rc;
+    return learned_fallback(dec, llr, nframes, fallback_iters, post, hard, ret, iters, synd, fallback, dec->d_count.p,
+                            dec->d_idx.p, (hipStream_t)stream);
 }
 
 int wl_decode_f32_host(wl_decoder* dec, float* rx, size_t nframes, int8_t* decoded, uint8_t* ret,

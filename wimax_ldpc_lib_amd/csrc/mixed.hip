@@ -61,6 +61,16 @@ __global__ void mixed_prep_kernel(uint32_t* counts, uint32_t* cursors, uint32_t*
     for (int i = threadIdx.x; i < kCodes + kNumZ; i += blockDim.x) tiles[i] = 0;
 }
 
+// fallback[g] = 0 for every frame of the learned graph: the re-queue kernels
+// then write 0 / 1 for their codes' frames, and the frames decoded by the
+// group kernels (normalised min-sum only) keep 0
+__global__ void zero_bytes_kernel(uint8_t* __restrict__ p, unsigned long long n)
+{
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
 // Reliability level of frame g: the mean |LLR| of its first kLevelSample LLRs
 // as float bits >> 19 (exponent and 4 mantissa bits: 16 steps per octave),
 // over 2^-8 .. 2^8; lower means, zero and NaN give 0, higher ones 255.  One
@@ -501,6 +511,8 @@ static int capture_decode(wl_mixed* m, int learned, int fallback_iters, const ui
     const unsigned sblocks = (unsigned)((n + 256 * kScatterPer - 1) / (256 * kScatterPer));
     hipLaunchKernelGGL(mixed_prep_kernel, dim3(1), dim3(256), 0, s, m->counts, m->cursors, m->lv_counts,
                        m->lv_cursors, m->tiles);
+    if (learned && fallback)
+        hipLaunchKernelGGL(zero_bytes_kernel, dim3(blocks), dim3(256), 0, s, fallback, n);
     // Reliability order: only where frames stop early (with fixed iterations
     // every frame costs the same).
     const bool ordered = (learned || (m->flags & WL_FLAG_EARLY_STOP)) && !(m->flags & WL_FLAG_MIXED_UNORDERED);
@@ -684,6 +696,8 @@ int wl_mixed_decode_learned_fallback_dev(wl_mixed* m, const uint8_t* cfg, const 
     if (!cfg || !frame_off || !llr) return mfail(WL_EINVAL, "cfg, frame_off and llr are required");
     if (nframes > 0xffffffffull) return mfail(WL_EINVAL, "nframes too large");
     if (fallback_iters < 0 || fallback_iters > 255) return mfail(WL_EINVAL, "fallback_iters out of range");
+    if (post && (const float*)post == llr)
+        return mfail(WL_EINVAL, "post is llr: re-queued frames are decoded again from the input LLRs");
     return run_graph(m, 1, fallback_iters, cfg, frame_off, llr, nframes, post, hard, ret, iters, synd, fallback,
                      (hipStream_t)stream);
 }

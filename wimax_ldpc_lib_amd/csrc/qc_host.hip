@@ -202,7 +202,10 @@ int wl_qc_from_alist(const char* text, size_t len, int z, wl_qc_code** out)
         return true;
     };
     long N, M, wc, wr;
-    if (!next(N) || !next(M) || !next(wc) || !next(wr) || N < 2 || M < 1 || M >= N || N > 32767 || wc < 1 || wr < 1)
+    // a column has at most M entries and a row at most N, so with N <= 32767
+    // every size below stays far from overflow
+    if (!next(N) || !next(M) || !next(wc) || !next(wr) || N < 2 || M < 1 || M >= N || N > 32767 || wc < 1 ||
+        wr < 1 || wc > M || wr > N)
         return qfail(WL_EINVAL, "alist: bad header");
     const size_t need = 4 + (size_t)N + (size_t)M + (size_t)N * wc + (size_t)M * wr;
     if (tok.size() < need) return qfail(WL_EINVAL, "alist: %zu numbers, header needs %zu", tok.size(), need);
@@ -211,8 +214,8 @@ int wl_qc_from_alist(const char* text, size_t len, int z, wl_qc_code** out)
     std::vector<std::vector<int>> rows((size_t)M);
     for (long m = 0; m < M; m++) {
         for (long j = 0; j < wr; j++) {
-            long v;
-            next(v);
+            long v = 0;
+            if (!next(v)) return qfail(WL_EINVAL, "alist: check list %ld ends early", m + 1);
             if (v < 0 || v > N) return qfail(WL_EINVAL, "alist: variable %ld out of range in check %ld", v, m + 1);
             if (v) rows[(size_t)m].push_back((int)v - 1);
         }

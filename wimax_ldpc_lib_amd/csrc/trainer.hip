@@ -337,6 +337,10 @@ int wl_trainer_load(wl_trainer* t, const char* path)
               fread(ve.data(), 4, t->nparam, fp) == (size_t)t->nparam;
     fclose(fp);
     if (!ok) return tfail(WL_EINVAL, "%s is not a checkpoint of this trainer's code and T", path);
+    // training frames are drawn at frame offset step * batch: another batch
+    // size would continue a different frame stream
+    if (hdr[7] != (uint32_t)t->batch)
+        return tfail(WL_EINVAL, "%s was written with batch %u, this trainer has batch %d", path, hdr[7], t->batch);
     TR_HIP(hipStreamSynchronize(t->stream));
     TR_HIP(hipMemcpy(t->theta, th.data(), t->nparam * 4, hipMemcpyHostToDevice));
     TR_HIP(hipMemcpy(t->mom, mo.data(), t->nparam * 4, hipMemcpyHostToDevice));
