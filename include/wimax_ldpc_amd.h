@@ -157,6 +157,25 @@ int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nfr
                                    uint8_t* ret, uint8_t* iters, uint16_t* synd,
                                    uint8_t* fallback, void* stream);
 
+/* Frames per call of wl_decode_learned_fallback_noalloc_dev. */
+#define WL_NOALLOC_MAX_FRAMES (1 << 20)
+
+/* wl_decode_learned_fallback_dev without any allocation (no counterpart in
+ * the reference).  It is for callers that cannot allocate device memory:
+ * code under HIP graph capture, tight serving loops, and harnesses that
+ * forbid hipMalloc.  Same arguments, kernels and results.  Its scratch is a
+ * static device array (4 MiB + 16 B on every GPU the library runs on, held
+ * until the process ends): the tile counter, and the list of failed frames
+ * where z is not 24 or 96.  At most WL_NOALLOC_MAX_FRAMES frames per call
+ * (WL_EINVAL beyond).  Calls on one GPU share that array, so calls that
+ * overlap on one GPU (two streams, or two host threads) race; calls
+ * serialised on one stream do not.  The decoder handle (offsets included)
+ * is set up beforehand as for the other entry points. */
+int wl_decode_learned_fallback_noalloc_dev(wl_decoder* dec, const float* llr, size_t nframes,
+                                           int fallback_iters, float* post, uint8_t* hard,
+                                           uint8_t* ret, uint8_t* iters, uint16_t* synd,
+                                           uint8_t* fallback, void* stream);
+
 /* ---------------------------------------------------------------- mixed batches */
 
 /* Frames of different (rate, z) decoded together (no counterpart in the

@@ -36,6 +36,7 @@ SIGNATURES = {
     "wl_compute_syndrome_host": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_uint)]),
     "wl_decoder_set_offsets": (C.c_int, [_P, _P, C.c_int]),
     "wl_decode_learned_fallback_dev": (C.c_int, [_P, _P, _SZ, C.c_int, _P, _P, _P, _P, _P, _P, _P]),
+    "wl_decode_learned_fallback_noalloc_dev": (C.c_int, [_P, _P, _SZ, C.c_int, _P, _P, _P, _P, _P, _P, _P]),
     "wl_mixed_create": (C.c_int, [C.c_int, C.c_uint, C.POINTER(_P)]),
     "wl_mixed_destroy": (C.c_int, [_P]),
     "wl_mixed_set_codes": (C.c_int, [_P, _P, C.c_int]),

@@ -213,6 +213,16 @@ class ldpc_decoder(ldpc):
                                                    _ptr(hard), _ptr(ret), _ptr(iters), _ptr(synd),
                                                    _ptr(fallback), _stream(stream)))
 
+    def decode_learned_fallback_noalloc_dev(self, llr, fallback_iters, post=None, hard=None, ret=None, iters=None,
+                                            synd=None, fallback=None, stream=None):
+        """decode_learned_fallback_dev without any device allocation (static
+        scratch; at most 1 << 20 frames per call; calls that overlap on one
+        GPU race)."""
+        F = llr.shape[0]
+        check(lib().wl_decode_learned_fallback_noalloc_dev(self._h, _ptr(llr), F, int(fallback_iters), _ptr(post),
+                                                           _ptr(hard), _ptr(ret), _ptr(iters), _ptr(synd),
+                                                           _ptr(fallback), _stream(stream)))
+
 
 class ldpc_encoder(ldpc):
     """include/ldpc_encoder.h:10-27 over wl_encoder."""

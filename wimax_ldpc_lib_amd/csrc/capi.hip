@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -431,6 +432,12 @@ int wl_decode_f32_dev(wl_decoder* dec, const float* llr, size_t nframes, float* 
 
 }  // extern "C"
 
+// Scratch of wl_decode_learned_fallback_noalloc_dev: [0..1] the counter,
+// [4 ..) the fallback list, in every device's copy of this array.  In host
+// code the name is only a handle for hipGetSymbolAddress: kernels are given
+// the address resolved on each device (device_scratch below).
+static __device__ __align__(256) uint32_t scratch[4 + (size_t)WL_NOALLOC_MAX_FRAMES];
+
 namespace {
 
 // Argument checks shared by the two learned-fallback entry points.  The
@@ -503,6 +510,26 @@ int learned_fallback(wl_decoder* dec, const float* llr, size_t nframes, int fall
                          s);
 }
 
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch[kMaxDevices] = {};
+
+// this device's copy of scratch (resolved on its first use)
+int device_scratch(uint32_t** out)
+{
+    int dev = 0;
+    WL_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(WL_ENOTSUP, "device %d beyond the %d the library tracks", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        WL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = static_cast<uint32_t*>(p);
+    }
+    *out = g_scratch[dev];
+    return WL_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -517,6 +544,21 @@ int wl_decode_learned_fallback_dev(wl_decoder* dec, const float* llr, size_t nfr
     if (!dec->entry->learned_fb && (rc = dec->d_idx.ensure(nframes))) return rc;
     return learned_fallback(dec, llr, nframes, fallback_iters, post, hard, ret, iters, synd, fallback, dec->d_count.p,
                             dec->d_idx.p, (hipStream_t)stream);
+}
+
+int wl_decode_learned_fallback_noalloc_dev(wl_decoder* dec, const float* llr, size_t nframes,
+                                           int fallback_iters, float* post, uint8_t* hard, uint8_t* ret,
+                                           uint8_t* iters, uint16_t* synd, uint8_t* fallback, void* stream)
+{
+    int rc = check_learned_fallback(dec, llr, nframes, fallback_iters, post, ret);
+    if (rc || nframes == 0) return rc;
+    if (nframes > (size_t)WL_NOALLOC_MAX_FRAMES)
+        return fail(WL_EINVAL, "nframes %zu exceeds the allocation-free limit of %d frames per call", nframes,
+                    WL_NOALLOC_MAX_FRAMES);
+    uint32_t* sc = nullptr;
+    if ((rc = device_scratch(&sc))) return rc;
+    return learned_fallback(dec, llr, nframes, fallback_iters, post, hard, ret, iters, synd, fallback, sc, sc + 4,
+                            (hipStream_t)stream);
 }
 
 int wl_decode_f32_host(wl_decoder* dec, float* rx, size_t nframes, int8_t* decoded, uint8_t* ret,
