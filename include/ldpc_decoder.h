@@ -21,6 +21,9 @@ namespace wimax_ldpc_lib {
             unsigned int m_max_iter;
             coderate     m_rate;
             wl_decoder*  m_handle;
+            unsigned int m_flags;
+
+            void recreate(unsigned int flags);
 
         public:
             ldpc_decoder();
@@ -46,6 +49,13 @@ namespace wimax_ldpc_lib {
 
             /* Extension: false = exactly max_iter iterations, no syndrome checks. */
             void set_early_stop(bool early_stop);
+
+            /* Extension: the message rule the reference selects at compile
+             * time (lib/ldpc_decoder.cc:17-21,215-238): RULE_NMS normalised
+             * min-sum (the default, as shipped), RULE_OMS offset min-sum,
+             * RULE_MS plain min-sum. */
+            enum message_rule { RULE_NMS = 0, RULE_OMS = 1, RULE_MS = 2 };
+            void set_message_rule(message_rule rule);
 
             wl_decoder* handle() { return m_handle; }
     };

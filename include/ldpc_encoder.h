@@ -20,6 +20,10 @@ namespace wimax_ldpc_lib {
 
         public:
             ldpc_encoder(coderate rate, unsigned int z_factor, unsigned int num_threads);
+            /* Extension: standard = true selects the back-substitution
+             * encoder that is valid for 3/4B as well (wl_encoder_create_standard);
+             * for the other rates the codewords are the same. */
+            ldpc_encoder(coderate rate, unsigned int z_factor, unsigned int num_threads, bool standard);
             ~ldpc_encoder();
 
             void encode_data(uint8_t* infoword, uint8_t* codeword);

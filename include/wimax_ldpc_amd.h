@@ -48,6 +48,12 @@ extern "C" {
 /* decoder flags */
 #define WL_FLAG_EARLY_STOP  0x1u  /* syndrome check before and after every iteration (as shipped) */
 #define WL_FLAG_MIXED_UNORDERED 0x2u  /* wl_mixed: no reliability order inside code buckets */
+/* message rule of wl_decoder (one at most; default: normalised min-sum, the
+ * reference as shipped, lib/ldpc_decoder.cc:17-18,217-224) */
+#define WL_FLAG_OMS 0x4u  /* offset min-sum s * max(min - 0.10, 0) in double: the
+                             reference built with OMS defined (:20,225-231) */
+#define WL_FLAG_MS  0x8u  /* plain min-sum s * min in float: the reference built
+                             with neither NMS nor OMS (:234-237) */
 
 typedef struct wl_decoder wl_decoder;
 typedef struct wl_encoder wl_encoder;
@@ -84,7 +90,12 @@ int wl_code_table(int rate, int z, int16_t* out, size_t out_len);
 /* ---------------------------------------------------------------- decoder */
 
 /* Replaces ldpc_decoder::ldpc_decoder(rate, z, max_iter, num_threads)
- * (include/ldpc_decoder.h:25-28, lib/ldpc_decoder.cc:27-59).  flags: WL_FLAG_*. */
+ * (include/ldpc_decoder.h:25-28, lib/ldpc_decoder.cc:27-59).  flags:
+ * WL_FLAG_EARLY_STOP, and at most one of WL_FLAG_OMS / WL_FLAG_MS (the
+ * offset and plain rules use the register-lean code shape whatever
+ * wl_decoder_set_variant chose; learned offsets, when set, replace the rule,
+ * and the learned fallback re-decodes with normalised min-sum; run-time QC
+ * codes: WL_ENOTSUP). */
 int wl_decoder_create(int rate, int z, int max_iter, unsigned flags, wl_decoder** out);
 int wl_decoder_destroy(wl_decoder* dec);
 
@@ -194,6 +205,8 @@ int wl_decode_learned_fallback_noalloc_dev(wl_decoder* dec, const float* llr, si
  * buffers and replayed afterwards.  Frames whose code id is invalid (or not
  * enabled, see wl_mixed_set_codes) are skipped. */
 typedef struct wl_mixed wl_mixed;
+/* flags: WL_FLAG_EARLY_STOP, WL_FLAG_MIXED_UNORDERED (WL_FLAG_OMS / MS:
+ * WL_ENOTSUP, mixed batches decode with normalised min-sum) */
 int wl_mixed_create(int max_iter, unsigned flags, wl_mixed** out);
 int wl_mixed_destroy(wl_mixed* m);
 /* Restrict the graph to the listed code ids (ncodes = 0: all 114). */
@@ -278,6 +291,16 @@ int wl_trainer_grad_dev(wl_trainer* tr, const float* llr, const uint8_t* bits, s
  * (include/ldpc_encoder.h:21, lib/ldpc_encoder.cc:9-17). */
 int wl_encoder_create(int rate, int z, wl_encoder** out);
 int wl_encoder_destroy(wl_encoder* enc);
+
+/* An encoder that produces valid codewords for every rate, 3/4B included (no
+ * counterpart in the reference, which cannot encode 3/4B: README.md:25, and
+ * whose encoder gives codewords that fail H there).  It solves the parity
+ * part by the general dual-diagonal back-substitution: the sum of all check
+ * rows gives P^h_mid p0 = XOR of the lambda blocks, then p_1, p_2, ... from
+ * the staircase.  Since the parity part is invertible, for the other five
+ * rates its codewords equal wl_encoder_create's bit for bit.  All wl_encode_*
+ * and wl_channel_dev calls accept it. */
+int wl_encoder_create_standard(int rate, int z, wl_encoder** out);
 
 /* ldpc_encoder::encode_data(uint8_t* info, uint8_t* codeword)
  * (include/ldpc_encoder.h:24, lib/ldpc_encoder.cc:50-105) over nframes

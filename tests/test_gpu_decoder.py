@@ -281,3 +281,58 @@ def test_nan_input_stays_in_its_frame(gpu, oracle_mod):
             assert np.isnan(g["post"][bad, 17])
             gk = {k: v[keep] for k, v in g.items()}
             _compare(gk, o, "NaN batch rate %d z %d early %d" % (rate, z, early))
+
+
+@pytest.mark.parametrize("rule", ["oms", "ms"])
+@pytest.mark.parametrize("rate", range(6))
+def test_offset_and_plain_min_sum_rules(gpu, oracle_mod, rule, rate):
+    """The reference's other compile-time message rules
+    (lib/ldpc_decoder.cc:20-21,225-238): offset min-sum (0.10, double) and
+    plain min-sum (float), bit-exact against the oracle's restatement of
+    each (itself pinned by an independent numpy decoder) for every rate at
+    six lifting sizes, both stop modes, every frames-per-workgroup setting at
+    one of them, and the special inputs of the edge-case test."""
+    import wimax_ldpc_lib_amd as W
+    variant = {"oms": oracle_mod.OMS, "ms": oracle_mod.MS}[rule]
+    ebn0 = {0: 1.8, 1: 2.4, 2: 2.4, 3: 2.8, 4: 2.8, 5: 3.4}[rate]
+    for z in (24, 32, 52, 68, 88, 96):
+        code = oracle_mod.Code(rate, z)
+        if rate == 4:
+            llr = np.random.default_rng(200 + z).normal(1.0, 2.0, (24, code.n)).astype(np.float32)
+        else:
+            llr, _ = noisy_llr(code, 24, ebn0, seed=3000 * rate + z)
+        for early in (True, False):
+            o = code.decode(llr, max_iter=8, early_stop=early, variant=variant)
+            dec = W.ldpc_decoder(rate, z, 8, early_stop=early, rule=rule)
+            _compare(_gpu_decode(gpu, dec, llr), o, "%s rate %d z %d early %d" % (rule, rate, z, early))
+    # frames per workgroup, partial workgroups
+    code = oracle_mod.Code(rate, 48)
+    llr, _ = noisy_llr(code, 37, ebn0, seed=77 + rate)
+    o = code.decode(llr, max_iter=6, early_stop=True, variant=variant)
+    dec = W.ldpc_decoder(rate, 48, 6, early_stop=True, rule=rule)
+    f = 1
+    while True:
+        try:
+            dec.frames_per_wg = f
+        except W.WLError:
+            break
+        for nf in (1, 37):
+            g = _gpu_decode(gpu, dec, np.ascontiguousarray(llr[:nf]))
+            _compare(g, {k: v[:nf] for k, v in o.items() if k != "seconds"}, "%s F=%d nf=%d" % (rule, f, nf))
+        f += 1
+    assert f > 2
+    # special inputs: clean codewords, |L| >= 10000, signed zeros, max_iter 0
+    if rate != 4:
+        code = oracle_mod.Code(rate, 96)
+        rng = np.random.default_rng(9)
+        cw = code.encode(rng.integers(0, 2, (2, code.k), dtype=np.uint8))
+        clean = np.where(cw == 1, 3.0, -3.0).astype(np.float32)
+        huge = np.where(rng.random((2, code.n)) < 0.5, 20000.0, -15000.0).astype(np.float32)
+        small = rng.normal(0, 0.05, (2, code.n)).astype(np.float32)   # |L| near the 0.10 offset
+        small[:, ::5] = -0.0
+        llr = np.concatenate([clean, huge, small])
+        for max_iter in (0, 3):
+            o = code.decode(llr, max_iter=max_iter, early_stop=True, variant=variant)
+            dec = W.ldpc_decoder(rate, 96, max_iter, early_stop=True, rule=rule)
+            _compare(_gpu_decode(gpu, dec, llr), o, "%s special max_iter %d" % (rule, max_iter))
+

@@ -139,3 +139,48 @@ def test_channel_statistics(gpu, oracle_mod):
     m1, m0 = x[b == 1].mean(), x[b == 0].mean()
     assert abs(m1 - 2 / s2) < 0.01 * 2 / s2 and abs(m0 + 2 / s2) < 0.01 * 2 / s2
     assert abs(x[b == 1].var() - 4 / s2) < 0.02 * 4 / s2
+
+
+@pytest.mark.parametrize("rate", range(6))
+def test_standard_encoder(gpu, oracle_mod, rate):
+    """wl_encoder_create_standard, the back-substitution encoder: at 3/4B its
+    codewords satisfy H at every z and equal the textbook GF(2) solution of
+    H_p p = H_u u (unique: the parity part is invertible); at the other five
+    rates they equal the reference encoder's (the oracle's) bit for bit.  Its
+    channel frames decode back to the codewords sent."""
+    import wimax_ldpc_lib_amd as W
+    from tests.test_oracle_pins import _gf2_parity
+    torch = gpu
+    for z in Z_VALUES:
+        code = oracle_mod.Code(rate, z)
+        rng = np.random.default_rng(900 + rate * 100 + z)
+        info = rng.integers(0, 2, (17, code.k), dtype=np.uint8)
+        enc = W.ldpc_encoder(rate, z, standard=True)
+        d_info = torch.from_numpy(info).cuda()
+        d_cw = torch.empty((17, code.n), dtype=torch.uint8, device="cuda")
+        enc.encode_dev(d_info, d_cw)
+        torch.cuda.synchronize()
+        cw = d_cw.cpu().numpy()
+        assert np.array_equal(cw[:, :code.k], info)
+        for f in range(17):
+            assert code.syndrome(cw[f].astype(np.int8)) == 0, (rate, z, f)
+        if rate != 4:
+            assert np.array_equal(cw, code.encode(info)), (rate, z)
+        elif z in (24, 48):
+            H = code.dense_h()
+            for f in range(3):
+                assert np.array_equal(cw[f, code.k:], _gf2_parity(H, code.k, info[f])), (z, f)
+    # BPSK / AWGN frames of the standard encoder at 6 dB decode to what was sent
+    z = 48
+    enc = W.ldpc_encoder(rate, z, standard=True)
+    F = 64
+    llr = torch.empty((F, enc.n), dtype=torch.float32, device="cuda")
+    bits = torch.empty((F, enc.n), dtype=torch.uint8, device="cuda")
+    enc.channel_dev(5, 0, F, 6.0, llr=llr, bits=bits)
+    dec = W.ldpc_decoder(rate, z, 10)
+    hard = torch.empty((F, enc.n // 8), dtype=torch.uint8, device="cuda")
+    ret = torch.empty(F, dtype=torch.uint8, device="cuda")
+    dec.decode_dev(llr, None, hard, ret)
+    torch.cuda.synchronize()
+    assert (ret.cpu().numpy() == 0).all()
+    assert np.array_equal(hard.cpu().numpy(), np.packbits(bits.cpu().numpy(), axis=1, bitorder="little"))

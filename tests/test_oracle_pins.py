@@ -145,6 +145,14 @@ def test_encoder_codewords_unique_solution(oracle_mod):
     c = oracle_mod.Code(4, 24)
     cw = c.encode(rng.integers(0, 2, (4, c.k), dtype=np.uint8))
     assert ((c.dense_h().astype(np.int64) @ cw.T) % 2).any(axis=0).all()
+    # ... although its parity part is invertible too: every info word has
+    # exactly one systematic codeword, which a correct 3/4B encoder must give
+    for z in (24, 48, 96):
+        c = oracle_mod.Code(4, z)
+        H = c.dense_h()
+        u = rng.integers(0, 2, c.k, dtype=np.uint8)
+        p = _gf2_parity(H, c.k, u)
+        assert not ((H.astype(np.int64) @ np.concatenate([u, p])) % 2).any()
 
 
 # ------------------------------------------------------------------ syndrome
@@ -240,10 +248,12 @@ def test_check_row_special_cases(oracle_mod):
 
 # ------------------------------------------------------------------ decoder
 
-def numpy_layered_decode(code, llr, max_iter, early_stop):
-    """Independent vectorised layered NMS decoder (numpy).  Check rule in the
-    'this bit equals the XOR of the others' form, min2 by masking the first
-    argmin, double-precision message step."""
+def numpy_layered_decode(code, llr, max_iter, early_stop, rule="nms"):
+    """Independent vectorised layered min-sum decoder (numpy).  Check rule in
+    the 'this bit equals the XOR of the others' form, min2 by masking the
+    first argmin; message step per the reference's three compile-time rules
+    (lib/ldpc_decoder.cc:215-238): "nms" s * (min * 0.815) and "oms"
+    s * max(min - 0.10, 0) in double, "ms" s * min in float."""
     H = code.dense_h().astype(np.int64)
     z = code.z
     rx = llr.astype(np.float32).copy()
@@ -268,7 +278,15 @@ def numpy_layered_decode(code, llr, max_iter, early_stop):
             nn = (~np.signbit(L)).astype(np.int64)
             others_odd = (nn.sum(axis=1, keepdims=True) - nn) % 2 == 1
             s = np.where(others_odd, 1.0, -1.0)
-            v = s * (mag.astype(np.float64) * 0.815)
+            if rule == "ms":
+                v = (s.astype(np.float32) * mag).astype(np.float32)
+                msgs[l] = v
+                rx[idx[l]] = L + v
+                continue
+            if rule == "nms":
+                v = s * (mag.astype(np.float64) * 0.815)
+            else:
+                v = s * np.maximum(mag.astype(np.float64) - 0.10, 0.0)
             msgs[l] = v.astype(np.float32)
             rx[idx[l]] = (L.astype(np.float64) + v).astype(np.float32)
         if early_stop and clean(rx):
@@ -276,17 +294,19 @@ def numpy_layered_decode(code, llr, max_iter, early_stop):
     return rx, max_iter, (0 if (not early_stop and clean(rx)) else (0 if max_iter == 0 else 1))
 
 
+@pytest.mark.parametrize("rule", ["nms", "oms", "ms"])
 @pytest.mark.parametrize("rate,z,ebn0", [(0, 24, 2.0), (5, 28, 3.5), (2, 52, 2.4), (4, 32, None)])
-def test_decoder_matches_independent_numpy(oracle_mod, rate, z, ebn0):
+def test_decoder_matches_independent_numpy(oracle_mod, rate, z, ebn0, rule):
     code = oracle_mod.Code(rate, z)
     if ebn0 is None:
         llr = np.random.default_rng(9).normal(0.5, 2.0, (6, code.n)).astype(np.float32)
     else:
         llr, _ = noisy_llr(code, 6, ebn0, seed=rate + z)
+    variant = {"nms": oracle_mod.NMS, "oms": oracle_mod.OMS, "ms": oracle_mod.MS}[rule]
     for early in (True, False):
-        o = code.decode(llr, 7, early)
+        o = code.decode(llr, 7, early, variant=variant)
         for f in range(llr.shape[0]):
-            post, it, ret = numpy_layered_decode(code, llr[f], 7, early)
+            post, it, ret = numpy_layered_decode(code, llr[f], 7, early, rule)
             assert np.array_equal(post.view(np.uint32), o["post"][f].view(np.uint32)), (f, early)
             assert it == o["iters"][f] and ret == o["ret"][f]
 

@@ -13,6 +13,8 @@ LIB_PATH = os.path.join(PKG, "lib", "libwimax_ldpc_amd.so")
 WL_OK, WL_EINVAL, WL_ENOMEM, WL_EDEVICE, WL_ENOTSUP = 0, -22, -12, -5, -95
 WL_FLAG_EARLY_STOP = 0x1
 WL_FLAG_MIXED_UNORDERED = 0x2
+WL_FLAG_OMS = 0x4
+WL_FLAG_MS = 0x8
 
 # every function the header declares: name -> (restype, argtypes)
 _P = C.c_void_p
@@ -63,6 +65,7 @@ SIGNATURES = {
     "wl_trainer_grad_dev": (C.c_int, [_P, _P, _P, _SZ, _P, _P, C.POINTER(C.c_double)]),
     "wl_encoder_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_P)]),
     "wl_encoder_destroy": (C.c_int, [_P]),
+    "wl_encoder_create_standard": (C.c_int, [C.c_int, C.c_int, C.POINTER(_P)]),
     "wl_encode_u8_dev": (C.c_int, [_P, _P, _SZ, _P, _P]),
     "wl_encode_u8_host": (C.c_int, [_P, _P, _SZ, _P]),
     "wl_channel_dev": (C.c_int, [_P, C.c_uint64, C.c_uint64, _SZ, C.c_float, C.c_float, _P, _P, _P, _P]),

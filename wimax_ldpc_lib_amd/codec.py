@@ -115,10 +115,15 @@ class qc_code:
 class ldpc_decoder(ldpc):
     """include/ldpc_decoder.h:12-34 over wl_decoder."""
 
-    def __init__(self, rate, z_factor, max_iter, num_threads=1, early_stop=True):
+    RULES = {"nms": 0, "oms": capi.WL_FLAG_OMS, "ms": capi.WL_FLAG_MS}
+
+    def __init__(self, rate, z_factor, max_iter, num_threads=1, early_stop=True, rule="nms"):
+        """rule: the reference's compile-time message rule
+        (lib/ldpc_decoder.cc:17-21,215-238): "nms" normalised min-sum (as
+        shipped), "oms" offset min-sum, "ms" plain min-sum."""
         super().__init__(rate, z_factor, num_threads)
         h = C.c_void_p()
-        flags = capi.WL_FLAG_EARLY_STOP if early_stop else 0
+        flags = (capi.WL_FLAG_EARLY_STOP if early_stop else 0) | self.RULES[rule]
         check(lib().wl_decoder_create(int(rate), int(z_factor), int(max_iter), flags, C.byref(h)))
         self._h = h
         self.max_iter = max_iter
@@ -227,10 +232,14 @@ class ldpc_decoder(ldpc):
 class ldpc_encoder(ldpc):
     """include/ldpc_encoder.h:10-27 over wl_encoder."""
 
-    def __init__(self, rate, z_factor, num_threads=1):
+    def __init__(self, rate, z_factor, num_threads=1, standard=False):
+        """standard=True: the general back-substitution encoder, valid for
+        3/4B too (wl_encoder_create_standard); the default is the
+        reference's encoder, whose 3/4B codewords fail H."""
         super().__init__(rate, z_factor, num_threads)
         h = C.c_void_p()
-        check(lib().wl_encoder_create(int(rate), int(z_factor), C.byref(h)))
+        fn = lib().wl_encoder_create_standard if standard else lib().wl_encoder_create
+        check(fn(int(rate), int(z_factor), C.byref(h)))
         self._h = h
 
     @classmethod

@@ -258,6 +258,7 @@ int wl_decoder_create(int rate, int z, int max_iter, unsigned flags, wl_decoder*
     if (!out) return fail(WL_EINVAL, "null output");
     *out = nullptr;
     if (max_iter < 0 || max_iter > 255) return fail(WL_EINVAL, "max_iter %d out of range 0..255", max_iter);
+    if ((flags & WL_FLAG_OMS) && (flags & WL_FLAG_MS)) return fail(WL_EINVAL, "WL_FLAG_OMS and WL_FLAG_MS together");
     wl_decoder* d = new (std::nothrow) wl_decoder();
     if (!d) return fail(WL_ENOMEM, "out of host memory");
     int rc = make_code(rate, z, d->code);
@@ -373,10 +374,12 @@ int wl_decoder_set_offsets(wl_decoder* dec, const float* offsets_host, int T)
     return WL_OK;
 }
 
+// learned: the learned offsets; otherwise the decoder's message rule, or
+// normalised min-sum whatever the rule with force_nms (the learned fallback)
 static int decode_launch(wl_decoder* dec, bool learned, int max_iter, bool early_stop,
                          const float* llr, size_t nframes, float* post, uint8_t* hard, uint8_t* ret,
                          uint8_t* iters, uint16_t* synd, const uint32_t* idx, const uint32_t* bucket,
-                         hipStream_t s)
+                         hipStream_t s, bool force_nms = false)
 {
     if (dec->generic) {
         if (learned || idx || bucket) return fail(WL_ENOTSUP, "run-time codes decode with normalised min-sum only");
@@ -413,7 +416,15 @@ static int decode_launch(wl_decoder* dec, bool learned, int max_iter, bool early
     a.max_iter = max_iter;
     a.early_stop = early_stop ? 1 : 0;
     a.n_offset_iters = dec->T;
-    hipError_t e = learned ? dec->entry->learned(a, s) : dec->entry->nms[dec->variant](a, s);
+    hipError_t e;
+    if (learned) {
+        e = dec->entry->learned(a, s);
+    } else if (!force_nms && (dec->flags & (WL_FLAG_OMS | WL_FLAG_MS))) {
+        a.rule = (dec->flags & WL_FLAG_OMS) ? 0 : 1;
+        e = dec->entry->alt(a, s);
+    } else {
+        e = dec->entry->nms[dec->variant](a, s);
+    }
     if (e != hipSuccess) return hip_fail(e, "decoder kernel launch");
     return WL_OK;
 }
@@ -507,7 +518,7 @@ int learned_fallback(wl_decoder* dec, const float* llr, size_t nframes, int fall
                        counter + 1, fallback);
     WL_HIP(hipGetLastError());
     return decode_launch(dec, false, fallback_iters, true, llr, nframes, post, hard, ret, iters, synd, idx, counter,
-                         s);
+                         s, true);
 }
 
 constexpr int kMaxDevices = 64;
@@ -637,6 +648,8 @@ int wl_decoder_create_qc(const wl_qc_code* q, int max_iter, unsigned flags, wl_d
     if (!out || !q) return fail(WL_EINVAL, "null argument");
     *out = nullptr;
     if (max_iter < 0 || max_iter > 255) return fail(WL_EINVAL, "max_iter %d out of range 0..255", max_iter);
+    if (flags & (WL_FLAG_OMS | WL_FLAG_MS))
+        return fail(WL_ENOTSUP, "run-time codes decode with normalised min-sum only");
     wl_decoder* d = new (std::nothrow) wl_decoder();
     if (!d) return fail(WL_ENOMEM, "out of host memory");
     d->code.info = q->info;
@@ -710,6 +723,18 @@ int wl_encoder_create(int rate, int z, wl_encoder** out)
     }
     e->desc = make_encode_desc(e->code);
     *out = e;
+    return WL_OK;
+}
+
+int wl_encoder_create_standard(int rate, int z, wl_encoder** out)
+{
+    int rc = wl_encoder_create(rate, z, out);
+    if (rc) return rc;
+    if (!wl::make_standard_encode_desc(rate, z, &(*out)->desc)) {
+        wl_encoder_destroy(*out);
+        *out = nullptr;
+        return fail(WL_ENOTSUP, "parity part of rate %d is not dual-diagonal", rate);
+    }
     return WL_OK;
 }
 

@@ -37,6 +37,36 @@ __device__ __forceinline__ void encode_frame(const EncodeDesc& d, const uint8_t*
         P[q] = 0;
     }
     const int mb = d.mb, X = d.x;
+    if (d.std_form) {
+        // Summing all check rows cancels every staircase block and the two
+        // h_top blocks of the first parity column: P^h_mid p0 = XOR of the
+        // lambda_q, so p0[t] = S[(t - h_mid) mod z].  Then row 0 gives
+        // p_1 = lambda_0 + P^h_top p0 and row q (1 <= q <= mb-2)
+        // p_{q+1} = lambda_q + p_q (+ P^h_mid p0 = S at q = x).
+        uint8_t S = 0;
+#pragma unroll
+        for (int q = 0; q < kMaxLayers; q++)
+            if (q < mb) S ^= V[q];
+        if (active) p0[t] = S;
+        __syncthreads();
+        if (active) {
+            int a0 = t - d.h_mid;
+            a0 += a0 < 0 ? z : 0;
+            int a1 = t + d.h_top - d.h_mid;
+            a1 += a1 < 0 ? z : 0;
+            a1 -= a1 >= z ? z : 0;
+            put(t, p0[a0]);
+            uint8_t p = V[0] ^ p0[a1];
+            put(z + t, p);
+#pragma unroll
+            for (int q = 1; q < kMaxLayers - 1; q++)
+                if (q <= mb - 2) {
+                    p = V[q] ^ p ^ (q == X ? S : (uint8_t)0);
+                    put((q + 1) * z + t, p);
+                }
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < kMaxLayers; q++)
         if (q == mb - 1) P[q] = V[q];

@@ -34,6 +34,7 @@ struct DecodeEntry {
     DecodeLaunch learned_fb;         // learned + in-kernel normalised min-sum re-queue
                                      // (persistent only; built for z = 24 and 96, else null)
     DecodeOccupancy occ_learned_fb;
+    DecodeLaunch alt;                // offset / plain min-sum (DecodeArgs::rule), register-lean shape
 };
 
 template <int RATE, int Z>
@@ -126,7 +127,8 @@ constexpr auto make_decode_table(std::integer_sequence<int, ZI...>)
                              w2_occupancy<RATE, z_of_index(ZI)>(), w4_occupancy<RATE, z_of_index(ZI)>()},
                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_LEARNED, 0, true>,
                             fb_launcher<RATE, z_of_index(ZI)>(),
-                            fb_occupancy<RATE, z_of_index(ZI)>()}...}};
+                            fb_occupancy<RATE, z_of_index(ZI)>(),
+                            &launch_tdmp<RATE, z_of_index(ZI), MODE_ALT, VAR_LEAN>}...}};
 }
 
 }  // namespace wl

@@ -127,6 +127,34 @@ EncodeDesc make_encode_desc_for(int rate, int z)
     return d;
 }
 
+bool make_standard_encode_desc(int rate, int z, EncodeDesc* out)
+{
+    EncodeDesc d = make_encode_desc_for(rate, z);
+    const LayerTable lt = make_layer_table(rate);
+    // the first parity column (block kb): shifts at base rows 0, x and mb-1
+    // only, equal at 0 and mb-1; block kb + j (j >= 1): shift-0 blocks at
+    // rows j-1 and j only
+    int h[kMaxLayers];
+    for (int l = 0; l < lt.nl; l++) {
+        h[l] = -1;
+        bool stair_ok = true;
+        for (int j = 0; j < lt.deg[l]; j++) {
+            const int b = lt.blk[l][j];
+            if (b == d.kb) h[l] = lift(rate, lt.p96[l][j], z);
+            if (b > d.kb && (lt.p96[l][j] != 0 || (b - d.kb != l && b - d.kb != l + 1))) stair_ok = false;
+        }
+        if (!stair_ok) return false;
+    }
+    for (int l = 1; l < lt.nl - 1; l++)
+        if (l != d.x && h[l] >= 0) return false;
+    if (h[0] < 0 || h[d.x] < 0 || h[0] != h[lt.nl - 1]) return false;
+    d.std_form = 1;
+    d.h_top = h[0];
+    d.h_mid = h[d.x];
+    *out = d;
+    return true;
+}
+
 hipError_t launch_encode(const EncodeDesc& d, const uint8_t* info, uint8_t* cw,
                          unsigned long long nframes, hipStream_t s)
 {

@@ -65,14 +65,14 @@ uint64_t ldpc::get_nanoseconds()
 
 // ----------------------------------------------------------------- decoder
 
-ldpc_decoder::ldpc_decoder() : m_max_iter(0), m_rate(HALFRATE), m_handle(nullptr) {}
+ldpc_decoder::ldpc_decoder() : m_max_iter(0), m_rate(HALFRATE), m_handle(nullptr), m_flags(WL_FLAG_EARLY_STOP) {}
 
 ldpc_decoder::ldpc_decoder(coderate rate, unsigned int z_factor, unsigned int max_iter,
                            unsigned int num_threads)
-    : ldpc(rate, z_factor, num_threads), m_max_iter(max_iter), m_rate(rate), m_handle(nullptr)
+    : ldpc(rate, z_factor, num_threads), m_max_iter(max_iter), m_rate(rate), m_handle(nullptr),
+      m_flags(WL_FLAG_EARLY_STOP)
 {
-    check(wl_decoder_create((int)rate, (int)z_factor, (int)max_iter, WL_FLAG_EARLY_STOP, &m_handle),
-          "wl_decoder_create");
+    check(wl_decoder_create((int)rate, (int)z_factor, (int)max_iter, m_flags, &m_handle), "wl_decoder_create");
 }
 
 ldpc_decoder::~ldpc_decoder()
@@ -122,17 +122,29 @@ void ldpc_decoder::decode_batch(const int8_t* rx, size_t nframes, int8_t* decode
     check(wl_decode_i8_host(m_handle, rx, nframes, decoded, ret, iters), "decode_batch");
 }
 
-void ldpc_decoder::set_early_stop(bool early_stop)
+// a new handle with other flags; frames per workgroup carry over
+void ldpc_decoder::recreate(unsigned int flags)
 {
-    // a new handle with the other flag; frames_per_wg carries over
+    if (!m_handle) throw std::runtime_error("decoder without a code");
     int f = 0;
-    check(wl_decoder_get_frames_per_wg(m_handle, &f), "set_early_stop");
+    check(wl_decoder_get_frames_per_wg(m_handle, &f), "decoder flags");
     wl_decoder* h = nullptr;
-    check(wl_decoder_create((int)m_rate, (int)m_z, (int)m_max_iter, early_stop ? WL_FLAG_EARLY_STOP : 0, &h),
-          "set_early_stop");
+    check(wl_decoder_create((int)m_rate, (int)m_z, (int)m_max_iter, flags, &h), "decoder flags");
     wl_decoder_set_frames_per_wg(h, f);
     wl_decoder_destroy(m_handle);
     m_handle = h;
+    m_flags = flags;
+}
+
+void ldpc_decoder::set_early_stop(bool early_stop)
+{
+    recreate(early_stop ? (m_flags | WL_FLAG_EARLY_STOP) : (m_flags & ~WL_FLAG_EARLY_STOP));
+}
+
+void ldpc_decoder::set_message_rule(message_rule rule)
+{
+    const unsigned base = m_flags & ~(WL_FLAG_OMS | WL_FLAG_MS);
+    recreate(rule == RULE_OMS ? (base | WL_FLAG_OMS) : (rule == RULE_MS ? (base | WL_FLAG_MS) : base));
 }
 
 // ----------------------------------------------------------------- encoder
@@ -141,6 +153,15 @@ ldpc_encoder::ldpc_encoder(coderate rate, unsigned int z_factor, unsigned int nu
     : ldpc(rate, z_factor, num_threads), m_handle(nullptr)
 {
     check(wl_encoder_create((int)rate, (int)z_factor, &m_handle), "wl_encoder_create");
+}
+
+ldpc_encoder::ldpc_encoder(coderate rate, unsigned int z_factor, unsigned int num_threads, bool standard)
+    : ldpc(rate, z_factor, num_threads), m_handle(nullptr)
+{
+    if (standard)
+        check(wl_encoder_create_standard((int)rate, (int)z_factor, &m_handle), "wl_encoder_create_standard");
+    else
+        check(wl_encoder_create((int)rate, (int)z_factor, &m_handle), "wl_encoder_create");
 }
 
 ldpc_encoder::~ldpc_encoder()

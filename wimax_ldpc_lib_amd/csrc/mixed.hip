@@ -368,6 +368,7 @@ int wl_mixed_create(int max_iter, unsigned flags, wl_mixed** out)
     if (!out) return mfail(WL_EINVAL, "null output");
     *out = nullptr;
     if (max_iter < 0 || max_iter > 255) return mfail(WL_EINVAL, "max_iter out of range 0..255");
+    if (flags & (WL_FLAG_OMS | WL_FLAG_MS)) return mfail(WL_ENOTSUP, "mixed batches decode with normalised min-sum");
     int dev = 0;
     hipDeviceProp_t prop;
     hipError_t e = hipGetDevice(&dev);

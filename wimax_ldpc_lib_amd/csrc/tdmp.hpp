@@ -51,6 +51,7 @@ struct DecodeArgs {
     int max_iter;
     int early_stop;            // 1: syndrome check before/after every iteration (as shipped)
     int n_offset_iters;        // T (learned mode)
+    int rule;                  // MODE_ALT: 0 offset min-sum, 1 plain min-sum
 };
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -122,7 +123,11 @@ constexpr int kMaxLdsBytes = 65536;
 // MODE_LEARNED_FB: learned decoding, then frames whose syndrome fails are
 // re-queued (per workgroup, in LDS) and decoded again from their input LLRs
 // with normalised min-sum in the same persistent kernel.
-enum { MODE_NMS = 0, MODE_LEARNED = 1, MODE_LEARNED_FB = 2 };
+// MODE_ALT: the reference's two other compile-time message rules
+// (lib/ldpc_decoder.cc:20-21,225-238), chosen per launch by DecodeArgs::rule:
+// offset min-sum s * max(min - 0.10, 0) in double, or plain min-sum s * min
+// in float.
+enum { MODE_NMS = 0, MODE_LEARNED = 1, MODE_LEARNED_FB = 2, MODE_ALT = 3 };
 
 // Code-shape variants (bit mask), all bit-identical in results:
 //   VAR_NOHOIST  recompute each slot's wrapped LDS address inside the layer
@@ -472,6 +477,40 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
                 msg[L][J] = __double2float_rn(dsg);
                 *atl(Lc, Jc) = __double2float_rn(__dadd_rn((double)v, dsg));
             });
+        } else if constexpr (MODE == MODE_ALT) {
+            if constexpr ((VAR & VAR_REREAD) != 0) asm volatile("" ::: "memory");
+            if (a.rule == 0) {
+                // OMS (:225-231): max((double)min - 0.10, 0.0) as std::max
+                // does, the row's sign parts as above, one rounding each
+                const double t1 = __dsub_rn((double)mn1, 0.10), t2 = __dsub_rn((double)mn2, 0.10);
+                const double d1 = t1 < 0.0 ? 0.0 : t1, d2 = t2 < 0.0 ? 0.0 : t2;
+                const uint32_t h1 = (uint32_t)__double2hiint(d1) | kw, h2 = (uint32_t)__double2hiint(d2) | kw;
+                const int l1 = __double2loint(d1), l2 = __double2loint(d2);
+                static_for<0, D>([&](auto Jc) {
+                    constexpr int J = decltype(Jc)::value;
+                    float v;
+                    if constexpr ((VAR & VAR_REREAD) != 0) v = *atl(Lc, Jc) - msg[L][J];
+                    else v = x[J];
+                    const bool e = fabsf(v) == mn1;
+                    const uint32_t hi = (e ? h2 : h1) ^ (__float_as_uint(v) & 0x80000000u);
+                    const double dsg = __hiloint2double((int)hi, e ? l2 : l1);
+                    msg[L][J] = __double2float_rn(dsg);
+                    *atl(Lc, Jc) = __double2float_rn(__dadd_rn((double)v, dsg));
+                });
+            } else {
+                // MS (:234-237): s * min and L + s * min in float
+                static_for<0, D>([&](auto Jc) {
+                    constexpr int J = decltype(Jc)::value;
+                    float v;
+                    if constexpr ((VAR & VAR_REREAD) != 0) v = *atl(Lc, Jc) - msg[L][J];
+                    else v = x[J];
+                    const bool e = fabsf(v) == mn1;
+                    const uint32_t sg = (__float_as_uint(v) ^ kw) & 0x80000000u;
+                    const float m = __uint_as_float(__float_as_uint(e ? mn2 : mn1) | sg);
+                    msg[L][J] = m;
+                    *atl(Lc, Jc) = __fadd_rn(v, m);
+                });
+            }
         } else {
             const int tt = t < a.n_offset_iters ? t : a.n_offset_iters - 1;
             const float* beta = a.offsets + ((size_t)tt * NL + L) * C;
@@ -753,10 +792,13 @@ hipError_t launch_tdmp(const DecodeArgs& a, hipStream_t stream)
     if (a.tile_counter && a.persistent_grid > 0 && (unsigned long long)a.persistent_grid < grid)
         grid = (unsigned long long)a.persistent_grid;
     if (grid > 0x7fffffffull) return hipErrorInvalidValue;
-    const bool ext = MODE == MODE_LEARNED_FB || a.frame_idx || a.bucket || a.frame_off || a.tile_counter;
+    // the alternative rules are compiled in the extended form only (it also
+    // decodes contiguous batches)
+    const bool ext = MODE == MODE_LEARNED_FB || MODE == MODE_ALT || a.frame_idx || a.bucket || a.frame_off ||
+                     a.tile_counter;
     if (ext)
         hipLaunchKernelGGL((tdmp_kernel<RATE, Z, MODE, VAR, true>), dim3((unsigned)grid), dim3(threads), lds, stream, a);
-    else if constexpr (MODE != MODE_LEARNED_FB)
+    else if constexpr (MODE != MODE_LEARNED_FB && MODE != MODE_ALT)
         hipLaunchKernelGGL((tdmp_kernel<RATE, Z, MODE, VAR, false>), dim3((unsigned)grid), dim3(threads), lds, stream, a);
     return hipGetLastError();
 }
