@@ -20,6 +20,7 @@
 #include "../wimax_ldpc_lib_amd/csrc/base_matrices.h"
 
 #define OR_MAXDEG 32
+#define OR_ENC_STACK 2048   /* parity checks of the largest 802.16e code: 1152 */
 
 /* Lifting rule of python_ldpc/ldpc_utils.py:99-110: floor(p z / 96) for every
  * rate except 2/3A, which uses p mod z; p <= 0 is kept as is. */
@@ -321,9 +322,16 @@ void or_encode(const or_code* code, const uint8_t* info, uint8_t* cw)
 {
     const int M = code->m, z = code->z, X = code->x, D = code->d, C = code->c;
     const int K = code->n - code->m;
-    uint8_t* P = (uint8_t*)calloc(M, 1);
-    uint8_t* V = (uint8_t*)calloc(M, 1);
-    uint8_t* Y = (uint8_t*)calloc(z, 1);
+    /* the reference's member buffers m_P, m_V (include/ldpc_encoder.h:15-16),
+     * cleared per frame (:55-56) */
+    uint8_t Pb[OR_ENC_STACK], Vb[OR_ENC_STACK], Yb[OR_ENC_STACK];
+    const int big = M > OR_ENC_STACK || z > OR_ENC_STACK;   /* run-time codes beyond 802.16e sizes */
+    uint8_t* P = big ? (uint8_t*)malloc((size_t)M) : Pb;
+    uint8_t* V = big ? (uint8_t*)malloc((size_t)M) : Vb;
+    uint8_t* Y = big ? (uint8_t*)malloc((size_t)z) : Yb;
+    memset(P, 0, (size_t)M);
+    memset(V, 0, (size_t)M);
+    memset(Y, 0, (size_t)z);
     for (int m = 0; m < M; m++) {                 /* compute_v, :35-47 */
         uint8_t value = 0;
         for (int n = 0; n < C; n++) {
@@ -340,7 +348,11 @@ void or_encode(const or_code* code, const uint8_t* info, uint8_t* cw)
     memcpy(cw, info, (size_t)K);                                            /* :90 */
     for (int i = 0; i < M; i++)                                             /* :93-103 */
         cw[K + i] = (i < z) ? P[i] : (uint8_t)(P[i] ^ P[(i + D) % z]);
-    free(P); free(V); free(Y);
+    if (big) {
+        free(P);
+        free(V);
+        free(Y);
+    }
 }
 
 static double now_s(void)
@@ -383,6 +395,20 @@ double or_decode_batch(const or_code* code, const or_dec_cfg* cfg, float* llr, s
         free(lmn);
         free(hloc);
     }
+    return now_s() - t0;
+}
+
+/* A frame-parallel batch of or_encode (OpenMP over frames; the reference
+ * encodes one frame per call, lib/encode_example.cc:131-158, with an OpenMP
+ * region inside compute_v, lib/ldpc_encoder.cc:34).  Returns the seconds of
+ * the timed region. */
+double or_encode_batch(const or_code* code, const uint8_t* info, uint8_t* cw, size_t nframes, int nthreads)
+{
+    if (nthreads <= 0) nthreads = or_omp_max_threads();
+    const size_t n = (size_t)code->n, k = (size_t)(code->n - code->m);
+    double t0 = now_s();
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (long f = 0; f < (long)nframes; f++) or_encode(code, info + (size_t)f * k, cw + (size_t)f * n);
     return now_s() - t0;
 }
 

@@ -91,6 +91,8 @@ unsigned or_decode_i8(const or_code* code, const or_dec_cfg* cfg, const int8_t* 
                       int8_t* hd, float* lmn, int* iters_out);
 
 void or_encode(const or_code* code, const uint8_t* info, uint8_t* cw);
+/* or_encode over nframes frames, OpenMP over frames; returns seconds */
+double or_encode_batch(const or_code* code, const uint8_t* info, uint8_t* cw, size_t nframes, int nthreads);
 
 /* Frame-parallel batch over nframes frames (OpenMP, one scratch per thread,
  * schedule(static)).  llr is [nframes][n] and is replaced by posteriors.

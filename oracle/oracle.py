@@ -60,6 +60,8 @@ def lib():
         L.or_decode_i8.argtypes = [C.POINTER(_Code), C.POINTER(_Cfg), P, P, P, C.POINTER(C.c_int)]
         L.or_decode_i8.restype = C.c_uint
         L.or_encode.argtypes = [C.POINTER(_Code), P, P]
+        L.or_encode_batch.argtypes = [C.POINTER(_Code), P, P, C.c_size_t, C.c_int]
+        L.or_encode_batch.restype = C.c_double
         L.or_decode_batch.argtypes = [C.POINTER(_Code), C.POINTER(_Cfg), P, C.c_size_t, P, P, P, P, C.c_int]
         L.or_decode_batch.restype = C.c_double
         L.or_learned_fwd_bwd.argtypes = [C.POINTER(_Code), P, C.c_int, P, P, C.c_size_t, P, P]
@@ -122,6 +124,13 @@ class Code:
         for f in range(info.shape[0]):
             lib().or_encode(C.byref(self._c), _p(info[f]), _p(cw[f]))
         return cw
+
+    def encode_batch(self, info, nthreads=0):
+        """info [F][k] uint8 -> (codewords [F][n], seconds of the OpenMP frame loop)."""
+        info = np.ascontiguousarray(info, dtype=np.uint8).reshape(-1, self.k)
+        cw = np.zeros((info.shape[0], self.n), dtype=np.uint8)
+        secs = lib().or_encode_batch(C.byref(self._c), _p(info), _p(cw), info.shape[0], nthreads)
+        return cw, secs
 
     def _cfg(self, max_iter, early_stop, variant, offsets):
         cfg = _Cfg(max_iter, int(early_stop), variant, None, 0)
