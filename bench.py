@@ -31,6 +31,8 @@ sys.path.insert(0, ROOT)
 # thread placement of the CPU baselines (read when an OpenMP runtime starts)
 os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
+# CPUs this process may use, read before an OpenMP runtime pins this thread
+USABLE_CPUS = len(os.sched_getaffinity(0))
 
 
 def cpu_model():
@@ -44,7 +46,7 @@ def cpu_model():
 
 
 def cpu_env(threads):
-    return {"cpu_model": cpu_model(), "threads": threads, "usable_cpus": len(os.sched_getaffinity(0)),
+    return {"cpu_model": cpu_model(), "threads": threads, "usable_cpus": USABLE_CPUS,
             "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES")}
 
 

@@ -11,16 +11,13 @@
 
 namespace wl {
 
-// Shared encoder body: info bytes (already 0/1) of frame slot fs in LDS at
-// u, P_0 scratch at p0 (z bytes); lane t < z.  Writes the parity bits through
-// put(i, bit) for i in [0, m).
-template <class Put>
-__device__ __forceinline__ void encode_frame(const EncodeDesc& d, const uint8_t* u, uint8_t* p0,
-                                             int t, bool active, Put&& put)
+// lambda_q[t] of lane t: the XOR over the info blocks of check row q t of
+// u[blk z + (t + s) mod z], u the frame's 0/1 info bits in LDS
+// (lib/ldpc_encoder.cc:29-48).  Run-time descriptor.
+__device__ __forceinline__ void encode_lambda(const EncodeDesc& d, const uint8_t* u, int t, bool active,
+                                              uint8_t (&V)[kMaxLayers])
 {
     const int z = d.z;
-    uint8_t V[kMaxLayers];
-    uint8_t P[kMaxLayers];
 #pragma unroll
     for (int q = 0; q < kMaxLayers; q++) {
         uint8_t v = 0;
@@ -34,8 +31,45 @@ __device__ __forceinline__ void encode_frame(const EncodeDesc& d, const uint8_t*
             }
         }
         V[q] = v;
-        P[q] = 0;
     }
+}
+
+// The same for the compiled code (RATE, Z): every slot's block and shift are
+// constants, so all of a lane's LDS reads can be in flight at once.
+template <int RATE, int Z>
+__device__ __forceinline__ void encode_lambda_ct(const uint8_t* u, int t, bool active, uint8_t (&V)[kMaxLayers])
+{
+    using CD = Code<RATE, Z>;
+    constexpr int KB = 24 - CD::NL;
+#pragma unroll
+    for (int q = 0; q < kMaxLayers; q++) V[q] = 0;
+    if (!active) return;
+    static_for<0, CD::NL>([&](auto Lc) {
+        constexpr int L = decltype(Lc)::value;
+        uint8_t v = 0;
+        static_for<0, CD::deg(L)>([&](auto Jc) {
+            constexpr int J = decltype(Jc)::value;
+            constexpr int b = CD::blk(L, J), s = CD::sh(L, J);
+            if constexpr (b < KB) {
+                const int p = t >= Z - s ? t + s - Z : t + s;
+                v ^= u[b * Z + p];
+            }
+        });
+        V[L] = v;
+    });
+}
+
+// Parity bits of the frame from its lambda blocks, lane t < z; P_0 scratch at
+// p0 (z bytes).  Writes bit i of the parity part through put(i, bit),
+// i in [0, m).  All threads of the workgroup call it (one barrier).
+template <class Put>
+__device__ __forceinline__ void encode_parity(const EncodeDesc& d, const uint8_t (&V)[kMaxLayers], uint8_t* p0,
+                                              int t, bool active, Put&& put)
+{
+    const int z = d.z;
+    uint8_t P[kMaxLayers];
+#pragma unroll
+    for (int q = 0; q < kMaxLayers; q++) P[q] = 0;
     const int mb = d.mb, X = d.x;
     if (d.std_form) {
         // Summing all check rows cancels every staircase block and the two
@@ -94,6 +128,18 @@ __device__ __forceinline__ void encode_frame(const EncodeDesc& d, const uint8_t*
         for (int q = 1; q < kMaxLayers; q++)
             if (q < mb) put(q * z + t, (uint8_t)(P[q] ^ corr));
     }
+}
+
+// Shared encoder body: info bytes (already 0/1) of frame slot fs in LDS at
+// u, P_0 scratch at p0 (z bytes); lane t < z.  Writes the parity bits through
+// put(i, bit) for i in [0, m).
+template <class Put>
+__device__ __forceinline__ void encode_frame(const EncodeDesc& d, const uint8_t* u, uint8_t* p0,
+                                             int t, bool active, Put&& put)
+{
+    uint8_t V[kMaxLayers];
+    encode_lambda(d, u, t, active, V);
+    encode_parity(d, V, p0, t, active, put);
 }
 
 // Source and channel of absolute frame g (Philox4x32-10 keyed by seed):

@@ -18,37 +18,90 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "channel_dev.hpp"
 #include "encoder.hpp"
 
 namespace wl {
 
-__global__ __launch_bounds__(256) void encode_kernel(EncodeDesc d, const uint8_t* __restrict__ info,
-                                                     uint8_t* __restrict__ cw,
-                                                     unsigned long long nframes)
+// One workgroup encodes a tile of F x passes frames: F frames at a time
+// (F z <= 512 lanes), `passes` times.  The tile's info words and codewords
+// are contiguous in HBM, so the systematic part is copied in vec-byte words
+// (16, 4 or 1: the info bytes as given, lib/ldpc_encoder.cc:90) while the 0/1
+// bits go to LDS; then every pass computes its frames' parity bits into LDS;
+// then the parity parts are stored in vec-byte words.  A tile of several
+// passes keeps a few loads per thread in flight.
+// RATE < 0: the run-time descriptor's loops; otherwise code (RATE, Z) with
+// its block columns and shifts as constants.
+template <int V>
+__device__ __forceinline__ void copy_info(const uint8_t* __restrict__ src, uint8_t* __restrict__ cw, uint8_t* u,
+                                          int k, int n, int nbytes, unsigned long long frame0)
 {
-    extern __shared__ uint8_t esm[];
-    const int z = d.z, k = d.k, n = d.n, F = d.frames_per_wg;
-    const unsigned long long frame0 = (unsigned long long)blockIdx.x * F;
+    using W = typename std::conditional<V == 16, uint4, typename std::conditional<V == 4, uint32_t, uint8_t>::type>::type;
+    const int nw = nbytes / V;
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) {
+        const W w = reinterpret_cast<const W*>(src)[i];
+        const int f = (V * i) / k, j = V * i - f * k;
+        *reinterpret_cast<W*>(cw + (frame0 + f) * n + j) = w;
+        // bytes != 0 -> 1: bit 7 of (b & 0x7f) + 0x7f, or of b itself
+        auto nz = [](uint32_t x) { return ((((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) >> 7) & 0x01010101u; };
+        if constexpr (V == 16) {
+            reinterpret_cast<uint4*>(u)[i] = make_uint4(nz(w.x), nz(w.y), nz(w.z), nz(w.w));
+        } else if constexpr (V == 4) {
+            reinterpret_cast<uint32_t*>(u)[i] = nz(w);
+        } else {
+            u[i] = w != 0;
+        }
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void store_parity(const uint8_t* par, uint8_t* __restrict__ cw, int k, int n, int m,
+                                             int Feff, unsigned long long frame0)
+{
+    using W = typename std::conditional<V == 16, uint4, typename std::conditional<V == 4, uint32_t, uint8_t>::type>::type;
+    const int mw = m / V;
+    for (int i = threadIdx.x; i < Feff * mw; i += blockDim.x) {
+        const int f = i / mw, q = i - f * mw;
+        reinterpret_cast<W*>(cw + (frame0 + f) * n + k)[q] = reinterpret_cast<const W*>(par)[f * mw + q];
+    }
+}
+
+template <int RATE, int Z>
+__global__ __launch_bounds__(512) void encode_kernel(EncodeDesc d, int F, int passes, const uint8_t* __restrict__ info,
+                                                     uint8_t* __restrict__ cw, unsigned long long nframes, int vec)
+{
+    extern __shared__ __align__(16) uint8_t esm[];
+    const int z = d.z, k = d.k, n = d.n, m = d.m;
+    const int T = F * passes;
+    const unsigned long long frame0 = (unsigned long long)blockIdx.x * T;
     const unsigned long long left = nframes - frame0;
-    const int Feff = left < (unsigned long long)F ? (int)left : F;
+    const int Feff = left < (unsigned long long)T ? (int)left : T;
     const int tid = threadIdx.x;
-    const int fs = tid / z, t = tid - fs * z;
-    const bool active = fs < Feff;
-    uint8_t* u = esm;                         // [F][k]
-    uint8_t* p0 = esm + (size_t)F * k;        // [F][z]
+    uint8_t* u = esm;                         // [T][k] info bits
+    uint8_t* par = esm + (size_t)T * k;       // [T][m] parity bits
+    uint8_t* p0 = par + (size_t)T * m;        // [T][z] P_0 scratch
     const uint8_t* src = info + frame0 * k;
-    uint8_t* dst = cw + frame0 * n;
-    for (int i = tid; i < Feff * k; i += blockDim.x) {
-        const uint8_t b = src[i];
-        u[i] = b != 0;
-        const int f = i / k, j = i - f * k;
-        dst[(size_t)f * n + j] = b;           // systematic part copied as is
+    if (vec == 16) copy_info<16>(src, cw, u, k, n, Feff * k, frame0);
+    else if (vec == 4) copy_info<4>(src, cw, u, k, n, Feff * k, frame0);
+    else copy_info<1>(src, cw, u, k, n, Feff * k, frame0);
+    __syncthreads();
+    const int t = tid % z;
+    for (int pass = 0; pass < passes; pass++) {
+        const int fs = pass * F + tid / z;
+        const bool active = tid < F * z && fs < Feff;
+        const int fsa = active ? fs : 0;
+        uint8_t Vq[kMaxLayers];
+        if constexpr (RATE < 0) encode_lambda(d, u + (size_t)fsa * k, t, active, Vq);
+        else encode_lambda_ct<RATE, Z>(u + (size_t)fsa * k, t, active, Vq);
+        encode_parity(d, Vq, p0 + (size_t)fsa * z, t, active,
+                      [&](int i, uint8_t bit) { par[(size_t)fs * m + i] = bit; });
     }
     __syncthreads();
-    const int fsa = active ? fs : 0;
-    encode_frame(d, u + (size_t)fsa * k, p0 + (size_t)fsa * z, t, active,
-                 [&](int i, uint8_t bit) { dst[(size_t)fs * n + k + i] = bit; });
+    if (vec == 16) store_parity<16>(par, cw, k, n, m, Feff, frame0);
+    else if (vec == 4) store_parity<4>(par, cw, k, n, m, Feff, frame0);
+    else store_parity<1>(par, cw, k, n, m, Feff, frame0);
 }
 
 __global__ __launch_bounds__(256) void channel_kernel(EncodeDesc d, unsigned long long seed,
@@ -117,6 +170,8 @@ EncodeDesc make_encode_desc_for(int rate, int z)
     d.x = bi.x;
     d.d = lift(rate, bi.d0, z);
     d.frames_per_wg = 256 / z > 0 ? 256 / z : 1;
+    d.compiled = 1;
+    d.rate = rate;
     for (int l = 0; l < lt.nl; l++) {
         d.deg[l] = (int8_t)lt.deg[l];
         for (int j = 0; j < lt.deg[l]; j++) {
@@ -155,15 +210,43 @@ bool make_standard_encode_desc(int rate, int z, EncodeDesc* out)
     return true;
 }
 
+using EncodeKernel = void (*)(EncodeDesc, int, int, const uint8_t*, uint8_t*, unsigned long long, int);
+
+template <int... I>
+constexpr auto make_encode_kernels(std::integer_sequence<int, I...>)
+{
+    struct Arr { EncodeKernel k[sizeof...(I)]; };
+    return Arr{{&encode_kernel<I / kNumZ, z_of_index(I % kNumZ)>...}};
+}
+// [rate * 19 + z index]
+constexpr auto kEncodeKernelArr = make_encode_kernels(std::make_integer_sequence<int, kNumRates * kNumZ>{});
+constexpr const EncodeKernel* kEncodeKernels = kEncodeKernelArr.k;
+
 hipError_t launch_encode(const EncodeDesc& d, const uint8_t* info, uint8_t* cw,
                          unsigned long long nframes, hipStream_t s)
 {
     if (nframes == 0) return hipSuccess;
-    const int F = d.frames_per_wg;
-    const unsigned long long grid = (nframes + F - 1) / F;
-    const size_t lds = (size_t)F * (d.k + d.z);
-    hipLaunchKernelGGL(encode_kernel, dim3((unsigned)grid), dim3(((F * d.z + 63) / 64) * 64), lds, s,
-                       d, info, cw, nframes);
+    // F frames per pass (F z <= 512 lanes), passes per tile while the tile's
+    // LDS (n + z bytes per frame) stays within 48 KiB
+    int F = 512 / d.z;
+    if (F < 1) F = 1;
+    const size_t per = (size_t)d.n + d.z;
+    while (F > 1 && (size_t)F * per > 49152) F--;
+    int passes = 1;
+    while (passes < 8 && (size_t)F * (passes + 1) * per <= 49152) passes++;
+    const unsigned long long T = (unsigned long long)F * passes;
+    const unsigned long long grid = (nframes + T - 1) / T;
+    if (grid > 0x7fffffffull) return hipErrorInvalidValue;
+    const size_t lds = (size_t)T * per;
+    const uintptr_t al = (uintptr_t)info | (uintptr_t)cw;
+    int vec = 1;
+    if (d.k % 16 == 0 && d.n % 16 == 0 && d.m % 16 == 0 && al % 16 == 0) vec = 16;
+    else if (d.k % 4 == 0 && d.n % 4 == 0 && d.m % 4 == 0 && al % 4 == 0) vec = 4;
+    EncodeKernel kern = &encode_kernel<-1, 0>;
+    if (d.compiled && d.rate >= 0 && d.rate < kNumRates && valid_z(d.z))
+        kern = kEncodeKernels[d.rate * kNumZ + index_of_z(d.z)];
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(((F * d.z + 63) / 64) * 64), lds, s, d, F, passes, info, cw,
+                       nframes, vec);
     return hipGetLastError();
 }
 

@@ -16,6 +16,9 @@ struct EncodeDesc {
     // first parity column's shifts h_top (base rows 0 and mb-1) and h_mid
     // (base row x); 0: the reference's parity prediction and correction
     int std_form, h_top, h_mid;
+    // compiled = 1: an 802.16e code (rate, z) whose encoder kernel is
+    // instantiated with its structure as constants; 0: run-time code
+    int compiled, rate;
     int8_t deg[kMaxLayers];
     int8_t blk[kMaxLayers][kMaxDeg];
     int8_t sh[kMaxLayers][kMaxDeg];
