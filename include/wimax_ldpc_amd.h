@@ -268,6 +268,11 @@ int wl_trainer_create(int rate, int z, int T, int batch, uint64_t seed, float lr
                       wl_trainer** out);
 int wl_trainer_destroy(wl_trainer* tr);
 int wl_trainer_set_hparams(wl_trainer* tr, float lr, float beta1, float beta2, float eps);
+/* per_iteration = 1: the loss is the mean over the T unrolled iterations of
+ * the cross-entropy of each iteration's posteriors (each with weight 1/T);
+ * 0 (default): the cross-entropy of the final posteriors.  Applies to
+ * wl_trainer_step and wl_trainer_grad_dev. */
+int wl_trainer_set_loss(wl_trainer* tr, int per_iteration);
 /* nsteps training steps (one captured hipGraph launch each); last_loss
  * (nullable) receives the mean loss of the last step's batch. */
 int wl_trainer_step(wl_trainer* tr, int nsteps, float ebn0_lo, float ebn0_hi, float* last_loss);

@@ -421,12 +421,32 @@ typedef struct {
     int parity;          /* 'sign' after the first pass */
 } or_rowstate;
 
+/* the cross-entropy term of posteriors P (bit 1 <=> P > 0) with weight w:
+ * adds w * (softplus(P) - b P) to *loss, and w * (sigmoid(P) - b) * scale to
+ * gP (or stores it, set = 1) */
+static void bce_term(const float* P, const uint8_t* bits, int n, double w, double scale, double* loss, float* gP,
+                     int set)
+{
+    for (int v = 0; v < n; v++) {
+        double x = P[v], b = bits[v] ? 1.0 : 0.0;
+        double sp = (x > 0 ? x : 0) + log1p(exp(-fabs(x)));
+        *loss += w * (sp - b * x);
+        double sg = 1.0 / (1.0 + exp(-x));
+        float g = (float)((sg - b) * scale * w);
+        gP[v] = set ? g : gP[v] + g;
+    }
+}
+
 double or_learned_fwd_bwd(const or_code* code, const float* offsets, int T,
                           const float* llr, const uint8_t* bits, size_t nframes,
-                          float* post_out, float* grad_off)
+                          float* post_out, float* grad_off, int per_iteration)
 {
     const int c = code->c, M = code->m, n = code->n, z = code->z;
     double loss = 0.0;
+    /* loss weights: the final posteriors only, or 1/T for every iteration's */
+    const int multi = per_iteration && T > 1;
+    const double w_final = multi ? 1.0 / T : 1.0, w_it = multi ? 1.0 / T : 0.0;
+    float* Pit = multi ? (float*)malloc(sizeof(float) * (size_t)(T - 1) * n) : NULL;
     float* P = (float*)malloc(sizeof(float) * n);
     float* Mmsg = (float*)malloc(sizeof(float) * (size_t)M * c);
     float* gP = (float*)malloc(sizeof(float) * n);
@@ -471,18 +491,15 @@ double or_learned_fwd_bwd(const or_code* code, const float* offsets, int T,
                     P[cn[j]] = L[j] + v;
                 }
             }
+            if (multi && t < T - 1) memcpy(Pit + (size_t)t * n, P, sizeof(float) * n);
         }
         if (post_out) memcpy(post_out + f * n, P, sizeof(float) * n);
         /* mean BCE over coded bits, bit 1 <=> P > 0: softplus(P) - b P */
-        for (int v = 0; v < n; v++) {
-            double x = P[v], b = bits[f * n + v] ? 1.0 : 0.0;
-            double sp = (x > 0 ? x : 0) + log1p(exp(-fabs(x)));
-            loss += sp - b * x;
-            double sg = 1.0 / (1.0 + exp(-x));
-            gP[v] = (float)((sg - b) * scale);
-        }
+        bce_term(P, bits + f * n, n, w_final, scale, &loss, gP, 1);
         memset(gM, 0, sizeof(float) * (size_t)M * c);
         for (int t = T - 1; t >= 0; t--) {
+            /* the loss term of iteration t enters at its posteriors */
+            if (multi && t < T - 1) bce_term(Pit + (size_t)t * n, bits + f * n, n, w_it, scale, &loss, gP, 0);
             for (int m = M - 1; m >= 0; m--) {
                 const int len = code->row_len[m];
                 const int16_t* cn = code->cn + (size_t)m * c;
@@ -517,6 +534,6 @@ double or_learned_fwd_bwd(const or_code* code, const float* offsets, int T,
         }
     }
     for (size_t i = 0; i < (size_t)T * code->mb * c; i++) grad_off[i] += (float)gacc[i];
-    free(P); free(Mmsg); free(gP); free(gM); free(st); free(gacc);
+    free(P); free(Mmsg); free(gP); free(gM); free(st); free(gacc); free(Pit);
     return loss;
 }

@@ -104,11 +104,13 @@ double or_decode_batch(const or_code* code, const or_dec_cfg* cfg, float* llr, s
 /* Learned offset min-sum training reference: forward over T unrolled
  * iterations (no early stop) saving the per-row selection state, then the
  * exact backward of the mean binary cross-entropy over the n coded bits
- * (bit 1 <=> posterior > 0).  grad_off [T][mb][c] is accumulated (+=).
- * Returns the loss summed over frames. */
+ * (bit 1 <=> posterior > 0), or with per_iteration the mean over the T
+ * iterations of the cross-entropy of each iteration's posteriors.
+ * grad_off [T][mb][c] is accumulated (+=).  Returns the loss summed over
+ * frames. */
 double or_learned_fwd_bwd(const or_code* code, const float* offsets, int T,
                           const float* llr, const uint8_t* bits, size_t nframes,
-                          float* post_out, float* grad_off);
+                          float* post_out, float* grad_off, int per_iteration);
 
 int or_omp_max_threads(void);
 

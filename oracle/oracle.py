@@ -64,7 +64,7 @@ def lib():
         L.or_encode_batch.restype = C.c_double
         L.or_decode_batch.argtypes = [C.POINTER(_Code), C.POINTER(_Cfg), P, C.c_size_t, P, P, P, P, C.c_int]
         L.or_decode_batch.restype = C.c_double
-        L.or_learned_fwd_bwd.argtypes = [C.POINTER(_Code), P, C.c_int, P, P, C.c_size_t, P, P]
+        L.or_learned_fwd_bwd.argtypes = [C.POINTER(_Code), P, C.c_int, P, P, C.c_size_t, P, P, C.c_int]
         L.or_learned_fwd_bwd.restype = C.c_double
         L.or_omp_max_threads.restype = C.c_int
         L.or_row_update.argtypes = [C.c_int, C.c_int, P, P, P]
@@ -166,7 +166,7 @@ class Code:
         r = lib().or_decode_i8(C.byref(self._c), C.byref(cfg), _p(bits), _p(hd), _p(lmn), C.byref(it))
         return r, hd, it.value
 
-    def learned_fwd_bwd(self, offsets, llr, bits):
+    def learned_fwd_bwd(self, offsets, llr, bits, per_iteration=False):
         offsets = np.ascontiguousarray(offsets, dtype=np.float32)
         T = offsets.shape[0]
         llr = np.ascontiguousarray(llr, dtype=np.float32).reshape(-1, self.n)
@@ -174,7 +174,7 @@ class Code:
         post = np.zeros_like(llr)
         g = np.zeros_like(offsets)
         loss = lib().or_learned_fwd_bwd(C.byref(self._c), _p(offsets), T, _p(llr), _p(bits),
-                                        llr.shape[0], _p(post), _p(g))
+                                        llr.shape[0], _p(post), _p(g), int(bool(per_iteration)))
         return loss, post, g
 
 

@@ -375,15 +375,17 @@ def test_learned_row_matches_brute_force(oracle_mod):
             assert np.array_equal(post, (L + msg).astype(np.float32))
 
 
-def test_learned_backward_matches_finite_differences(oracle_mod):
+@pytest.mark.parametrize("per_iteration", [False, True])
+def test_learned_backward_matches_finite_differences(oracle_mod, per_iteration):
     """The oracle's exact backward against central differences of its own
-    forward loss (double accumulation) on a small code."""
+    forward loss (double accumulation) on a small code, for the loss on the
+    final posteriors and for the mean of every iteration's loss."""
     code = oracle_mod.Code(1, 24)
     llr, cw = noisy_llr(code, 24, 2.0, seed=21)
     T = 3
     rng = np.random.default_rng(22)
     off = rng.uniform(0.05, 0.4, (T, code.mb, code.c)).astype(np.float32)
-    loss, _, g = code.learned_fwd_bwd(off, llr, cw)
+    loss, _, g = code.learned_fwd_bwd(off, llr, cw, per_iteration)
     N = llr.shape[0] * code.n
     checked = 0
     for idx in np.argsort(-np.abs(g).ravel())[:12]:
@@ -394,12 +396,22 @@ def test_learned_backward_matches_finite_differences(oracle_mod):
         op, om = off.copy(), off.copy()
         op[t, l, j] += h
         om[t, l, j] -= h
-        lp, _, _ = code.learned_fwd_bwd(op, llr, cw)
-        lm, _, _ = code.learned_fwd_bwd(om, llr, cw)
+        lp, _, _ = code.learned_fwd_bwd(op, llr, cw, per_iteration)
+        lm, _, _ = code.learned_fwd_bwd(om, llr, cw, per_iteration)
         fd = (lp - lm) / (2 * h) / N
         assert abs(fd - g[t, l, j]) <= 0.05 * abs(g[t, l, j]) + 1e-7, (t, l, j, fd, g[t, l, j])
         checked += 1
     assert checked >= 8
+    if per_iteration:
+        # the per-iteration loss is the mean of the losses of the unrolled
+        # decoders of depth 1 .. T (their final losses), and so its gradient
+        losses, grads = zip(*[(lv, gv) for lv, _, gv in (code.learned_fwd_bwd(off[:t + 1], llr, cw)
+                                                          for t in range(T))])
+        assert abs(loss - np.mean(losses)) <= 1e-9 * abs(loss)
+        want = np.zeros_like(g)
+        for t in range(T):
+            want[:t + 1] += grads[t] / T
+        assert np.abs(g - want).max() <= 1e-6 * np.abs(want).max()
 
 
 def test_learned_forward_equals_decoder_variant(oracle_mod):

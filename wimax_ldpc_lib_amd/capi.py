@@ -55,6 +55,7 @@ SIGNATURES = {
                                      C.POINTER(_P)]),
     "wl_trainer_destroy": (C.c_int, [_P]),
     "wl_trainer_set_hparams": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "wl_trainer_set_loss": (C.c_int, [_P, C.c_int]),
     "wl_trainer_step": (C.c_int, [_P, C.c_int, C.c_float, C.c_float, C.POINTER(C.c_float)]),
     "wl_trainer_num_params": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "wl_trainer_get_offsets": (C.c_int, [_P, _P]),

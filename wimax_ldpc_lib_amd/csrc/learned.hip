@@ -5,13 +5,16 @@
 // (one learned offset per nonzero base-matrix block (l, j) per iteration t),
 // mag = min1, or min2 for the row's first strict argmin.  Loss: mean binary
 // cross-entropy of the final posteriors over all n coded bits (bit 1 <=>
-// posterior > 0, the reference's sign convention).
+// posterior > 0, the reference's sign convention), or the weighted sum of
+// that cross-entropy after every unrolled iteration (the gradient of an
+// earlier iteration's term enters the reverse sweep at that iteration).
 //
 //   learned_fwd_kernel  forward over T iterations; per (frame, t, row) saves
 //                       argmin, second argmin, the sign bits of L and the
 //                       mask of slots whose clip was active; writes
-//                       dLoss/dP of the final posteriors and per-workgroup
-//                       loss sums
+//                       dLoss/dP of the final posteriors (and, with the
+//                       per-iteration loss, of the posteriors after each
+//                       earlier iteration) and per-workgroup loss sums
 //   learned_bwd_kernel  reverse sweep (t = T-1..0, layers in reverse); the
 //                       gradient of a slot's magnitude flows to the argmin
 //                       (or, for the argmin slot, the second argmin) with the
@@ -59,6 +62,7 @@ __global__ __launch_bounds__(kTrainThreads) void learned_fwd_kernel(TrainArgs a)
     static_for<0, NL>([&](auto Lc) {
         static_for<0, C>([&](auto Jc) { msg[decltype(Lc)::value][decltype(Jc)::value] = 0.0f; });
     });
+    double lsum_it = 0.0;   // this thread's part of the per-iteration loss terms
     for (int t = 0; t < a.T; t++) {
         static_for<0, NL>([&](auto Lc) {
             constexpr int L = decltype(Lc)::value;
@@ -100,16 +104,29 @@ __global__ __launch_bounds__(kTrainThreads) void learned_fwd_kernel(TrainArgs a)
             }
             __syncthreads();
         });
+        if (a.gpost_it && t < a.T - 1) {
+            // the weighted loss term of iteration t and its dLoss/dP
+            for (int i = tid; i < Feff * N; i += blockDim.x) {
+                const int fi = i / N, v = i - fi * N;
+                const float p = rxall[i];
+                const float b = a.bits[frame0 * N + i] ? 1.0f : 0.0f;
+                const float sp = fmaxf(p, 0.0f) + log1pf(expf(-fabsf(p)));
+                lsum_it += (double)(a.w_it * (sp - b * p));
+                const float sg = 1.0f / (1.0f + expf(-p));
+                a.gpost_it[((frame0 + fi) * (size_t)(a.T - 1) + t) * N + v] = (sg - b) * a.grad_scale * a.w_it;
+            }
+            __syncthreads();   // rxall is read here and rewritten by the next iteration
+        }
     }
     // loss and dLoss/dP of the final posteriors
-    double lsum = 0.0;
+    double lsum = lsum_it;
     for (int i = tid; i < Feff * N; i += blockDim.x) {
         const float p = rxall[i];
         const float b = a.bits[frame0 * N + i] ? 1.0f : 0.0f;
         const float sp = fmaxf(p, 0.0f) + log1pf(expf(-fabsf(p)));
-        lsum += (double)(sp - b * p);
+        lsum += (double)(a.w_final * (sp - b * p));
         const float sg = 1.0f / (1.0f + expf(-p));
-        a.gpost[frame0 * N + i] = (sg - b) * a.grad_scale;
+        a.gpost[frame0 * N + i] = (sg - b) * a.grad_scale * a.w_final;
         if (a.post) a.post[frame0 * N + i] = p;
     }
     float ls = (float)lsum;
@@ -153,6 +170,14 @@ __global__ __launch_bounds__(kTrainThreads) void learned_bwd_kernel(TrainArgs a)
         static_for<0, C>([&](auto Jc) { gm[decltype(Lc)::value][decltype(Jc)::value] = 0.0f; });
     });
     for (int t = a.T - 1; t >= 0; t--) {
+        if (a.gpost_it && t < a.T - 1) {
+            // the loss term of iteration t joins the gradient of its posteriors
+            for (int i = tid; i < Feff * N; i += blockDim.x) {
+                const int fi = i / N, v = i - fi * N;
+                gall[i] += a.gpost_it[((frame0 + fi) * (size_t)(a.T - 1) + t) * N + v];
+            }
+            __syncthreads();
+        }
         float db[NL][C];
         static_for<0, NL>([&](auto Lc) {
             static_for<0, C>([&](auto Jc) { db[decltype(Lc)::value][decltype(Jc)::value] = 0.0f; });

@@ -21,6 +21,11 @@ struct TrainArgs {
     int frames_per_wg;
     int T;
     float grad_scale;        // 1 / (nframes * n): the loss is a mean
+    // per-iteration loss: the weighted sum of the BCE after every unrolled
+    // iteration; w_final weights the last one, w_it each of the T-1 others
+    // (w_it = 0, gpost_it unused: the final posteriors only)
+    float* gpost_it;         // [nframes][T-1][n] dLoss/dP after iterations 0 .. T-2 (nullable)
+    float w_final, w_it;
 };
 
 using TrainLaunch = hipError_t (*)(const TrainArgs&, hipStream_t);

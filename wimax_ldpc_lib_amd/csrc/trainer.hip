@@ -80,6 +80,12 @@ struct wl_trainer {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     float g_lo = 0.0f, g_hi = 0.0f;   // Eb/N0 range the graph was captured with
+    // per-iteration loss: dLoss/dP after iterations 0 .. T-2, training batch
+    // and wl_trainer_grad_dev scratch
+    int per_iter = 0;
+    float* gpost_it = nullptr;
+    float* ggpost_it = nullptr;
+    size_t gcap_it = 0;
 };
 
 namespace {
@@ -89,10 +95,19 @@ void free_all(wl_trainer* t)
     if (t->gexec) (void)hipGraphExecDestroy(t->gexec);
     if (t->graph) (void)hipGraphDestroy(t->graph);
     void* ps[] = {t->theta, t->mom, t->vel, t->grad, t->llr, t->gpost, t->loss_part, t->dbeta_part,
-                  t->loss_sum, t->bits, t->state, t->step_dev, t->gstate, t->ggpost, t->gloss, t->gdb};
+                  t->loss_sum, t->bits, t->state, t->step_dev, t->gstate, t->ggpost, t->gloss, t->gdb,
+                  t->gpost_it, t->ggpost_it};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (t->stream) (void)hipStreamDestroy(t->stream);
+}
+
+// loss weights: the final posteriors only, or 1/T for each iteration's
+void set_loss_weights(const wl_trainer* t, wl::TrainArgs& a, float* gpost_it)
+{
+    a.gpost_it = (t->per_iter && t->T > 1) ? gpost_it : nullptr;
+    a.w_final = a.gpost_it ? 1.0f / (float)t->T : 1.0f;
+    a.w_it = a.gpost_it ? 1.0f / (float)t->T : 0.0f;
 }
 
 int enqueue_step(wl_trainer* t, hipStream_t s)
@@ -113,6 +128,7 @@ int enqueue_step(wl_trainer* t, hipStream_t s)
     a.frames_per_wg = t->entry->frames_per_wg;
     a.T = t->T;
     a.grad_scale = 1.0f / ((float)t->batch * (float)t->n);
+    set_loss_weights(t, a, t->gpost_it);
     if ((e = t->entry->fwd(a, s)) != hipSuccess) return tfail(WL_EDEVICE, "forward: %s", hipGetErrorString(e));
     if ((e = t->entry->bwd(a, s)) != hipSuccess) return tfail(WL_EDEVICE, "backward: %s", hipGetErrorString(e));
     if ((e = wl::launch_reduce_partials(t->dbeta_part, t->nwg, t->nparam, t->grad, s)) != hipSuccess)
@@ -224,6 +240,24 @@ int wl_trainer_set_hparams(wl_trainer* t, float lr, float beta1, float beta2, fl
     t->b2 = beta2;
     t->eps = eps;
     if (t->gexec) {   // captured launches hold the old values
+        (void)hipGraphExecDestroy(t->gexec);
+        (void)hipGraphDestroy(t->graph);
+        t->gexec = nullptr;
+        t->graph = nullptr;
+    }
+    return WL_OK;
+}
+
+int wl_trainer_set_loss(wl_trainer* t, int per_iteration)
+{
+    if (!t) return tfail(WL_EINVAL, "null trainer");
+    if (per_iteration != 0 && per_iteration != 1) return tfail(WL_EINVAL, "per_iteration is 0 or 1");
+    if (per_iteration && t->T > 1 && !t->gpost_it) {
+        int rc = dalloc(&t->gpost_it, (size_t)t->batch * (t->T - 1) * t->n);
+        if (rc) return rc;
+    }
+    t->per_iter = per_iteration;
+    if (t->gexec) {   // the captured step holds the old loss
         (void)hipGraphExecDestroy(t->gexec);
         (void)hipGraphDestroy(t->graph);
         t->gexec = nullptr;
@@ -399,6 +433,15 @@ int wl_trainer_grad_dev(wl_trainer* t, const float* llr, const uint8_t* bits, si
     a.frames_per_wg = F;
     a.T = t->T;
     a.grad_scale = (float)(1.0 / ((double)nframes * t->n));
+    if (t->per_iter && t->T > 1 && nframes > t->gcap_it) {
+        if (t->ggpost_it) (void)hipFree(t->ggpost_it);
+        t->ggpost_it = nullptr;
+        t->gcap_it = 0;
+        int rc = dalloc(&t->ggpost_it, nframes * (t->T - 1) * t->n);
+        if (rc) return rc;
+        t->gcap_it = nframes;
+    }
+    set_loss_weights(t, a, t->ggpost_it);
     hipError_t e;
     if ((e = t->entry->fwd(a, t->stream)) != hipSuccess) return tfail(WL_EDEVICE, "forward: %s", hipGetErrorString(e));
     if ((e = t->entry->bwd(a, t->stream)) != hipSuccess) return tfail(WL_EDEVICE, "backward: %s", hipGetErrorString(e));

@@ -51,6 +51,11 @@ class Trainer:
     def set_hparams(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
         check(lib().wl_trainer_set_hparams(self._h, float(lr), float(beta1), float(beta2), float(eps)))
 
+    def set_loss(self, per_iteration):
+        """True: mean over the T iterations of each iteration's cross-entropy;
+        False: the final posteriors' cross-entropy (default)."""
+        check(lib().wl_trainer_set_loss(self._h, 1 if per_iteration else 0))
+
     def global_step(self):
         s = C.c_uint64()
         check(lib().wl_trainer_get_step(self._h, C.byref(s)))

@@ -3,8 +3,10 @@
 Training: rate 2/3A, z = 96 by default, T = 5 unrolled iterations, one offset
 per nonzero base-matrix block per iteration initialised to 0.15, batches of
 4096 fresh GPU-generated frames (seeded random bits, encoder, BPSK/AWGN with
-Eb/N0 uniform in [1, 4] dB), 20000 Adam steps at lr 1e-3, mean binary
-cross-entropy on the coded bits.
+Eb/N0 uniform in [1, 4] dB, or --train-lo / --train-hi), 20000 Adam steps at
+lr 1e-3, mean binary cross-entropy on the coded bits (of the final
+posteriors, or with --per-iteration the mean over every unrolled
+iteration's).
 
 Evaluation: fresh frames (another seed) at Eb/N0 = 1.0 .. 4.0 dB in 0.25 dB
 steps, `--eval-frames` per point, decoded by
@@ -135,6 +137,8 @@ def main():
     ap.add_argument("--init", type=float, default=0.15)
     ap.add_argument("--train-lo", type=float, default=1.0)
     ap.add_argument("--train-hi", type=float, default=4.0)
+    ap.add_argument("--per-iteration", action="store_true",
+                    help="loss: mean of every unrolled iteration's cross-entropy (default: the final one)")
     ap.add_argument("--seed", type=int, default=20240101)
     ap.add_argument("--eval-frames", type=int, default=1 << 20)
     ap.add_argument("--eval-lo", type=float, default=1.0)
@@ -161,6 +165,7 @@ def main():
         offsets = np.array(json.load(open(a.offsets_in))["offsets"], dtype=np.float32)
     else:
         tr = Trainer(a.rate, a.z, T=a.T, batch=a.batch, seed=a.seed, lr=a.lr, init_offset=a.init)
+        tr.set_loss(a.per_iteration)
         t0 = time.time()
         losses = []
         done = 0
@@ -175,7 +180,8 @@ def main():
         if a.checkpoint:
             tr.save(a.checkpoint)
         result["training"] = {"steps": a.steps, "batch": a.batch, "lr": a.lr, "init_offset": a.init,
-                              "ebn0_range_db": [a.train_lo, a.train_hi], "seconds": train_s,
+                              "ebn0_range_db": [a.train_lo, a.train_hi], "per_iteration_loss": a.per_iteration,
+                              "seconds": train_s,
                               "ms_per_step": train_s * 1e3 / max(1, a.steps), "loss_curve": losses}
         if a.offsets_out:
             json.dump({"rate": a.rate, "z": a.z, "T": a.T, "offsets": offsets.tolist(),
