@@ -215,6 +215,10 @@ int wl_mixed_set_codes(wl_mixed* m, const int* codes, int ncodes);
  * shape variant and persistent grid of the kernel that decodes all rates of
  * that z. */
 int wl_mixed_get_launch(const wl_mixed* m, int rate, int z, int* frames_per_wg, int* variant, int* grid);
+/* Frames per workgroup of the kernel that decodes the codes of lifting size z
+ * (0: the value the autotuner measured on mixed batches, compiled in).
+ * Results do not depend on it. */
+int wl_mixed_set_group_frames_per_wg(wl_mixed* m, int z, int frames_per_wg);
 /* frame_off [nframes] = exclusive prefix sum of n over cfg [nframes] (device). */
 int wl_mixed_offsets_dev(wl_mixed* m, const uint8_t* cfg, size_t nframes, uint64_t* frame_off, void* stream);
 int wl_mixed_decode_dev(wl_mixed* m, const uint8_t* cfg, const uint64_t* frame_off, const float* llr,

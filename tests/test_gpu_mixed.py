@@ -191,6 +191,40 @@ def test_mixed_reliability_order_leaves_results_unchanged(gpu):
         assert ret[g] == r1.cpu().numpy()[0] and iters[g] == i1.cpu().numpy()[0], g
 
 
+def test_mixed_group_frames_per_workgroup_leave_results_unchanged(gpu):
+    """Every frames-per-workgroup value of the group kernels (1, the largest
+    that fits, and values between) decodes bit-identically; values outside
+    1 .. largest are rejected, 0 restores the compiled default."""
+    import wimax_ldpc_lib_amd as W
+    from wimax_ldpc_lib_amd.mixed import MixedDecoder
+    torch = gpu
+    codes = [r * 19 + zi for r in (0, 1, 2, 3, 5) for zi in (0, 1, 8, 18)]
+    md, cfg_np, cfg, off, offs, n_of, llr, bits = _setup(torch, 5000, 43, codes)
+    want = [t.cpu().numpy() for t in _mixed_decode(torch, md, cfg, off, llr)]
+    for z in (24, 28, 56, 96):
+        default = md.launch(0, z)[0]
+        fmax = 1
+        while True:
+            try:
+                md.set_group_frames_per_wg(z, fmax + 1)
+                fmax += 1
+            except W.WLError:
+                break
+        assert fmax >= default >= 1
+        with pytest.raises(W.WLError):
+            md.set_group_frames_per_wg(z, -1)
+        for f in sorted({1, 2, (fmax + 1) // 2, fmax}):
+            md.set_group_frames_per_wg(z, f)
+            assert md.launch(3, z)[0] == f
+            got = [t.cpu().numpy() for t in _mixed_decode(torch, md, cfg, off, llr)]
+            for a, b in zip(got, want):
+                assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), (z, f)
+        md.set_group_frames_per_wg(z, 0)
+        assert md.launch(0, z)[0] == default
+    fresh = MixedDecoder(max_iter=10, early_stop=True)
+    assert all(fresh.launch(0, z)[0] == md.launch(0, z)[0] for z in range(24, 97, 4))
+
+
 def _gather_index(torch, offs, sel, n):
     """[len(sel), n] element indices of frames sel of a ragged buffer (device)."""
     base = torch.from_numpy(offs[sel].astype(np.int64)).cuda()

@@ -113,7 +113,9 @@ def build(force=False, jobs=None, verbose=False):
         for out in ex.map(_compile, cmds):
             if verbose and out.strip():
                 sys.stderr.write(out)
-    if force or cmds or not os.path.exists(LIB):
+    # relink also when an object is newer than the library (a build cut off
+    # between its compiles and the link leaves the objects current)
+    if force or cmds or not os.path.exists(LIB) or any(os.path.getmtime(o) > os.path.getmtime(LIB) for o in objs):
         tmp = LIB + ".tmp"
         _compile([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
         os.replace(tmp, LIB)

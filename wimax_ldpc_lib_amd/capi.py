@@ -44,6 +44,7 @@ SIGNATURES = {
     "wl_mixed_set_codes": (C.c_int, [_P, _P, C.c_int]),
     "wl_mixed_get_launch": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
+    "wl_mixed_set_group_frames_per_wg": (C.c_int, [_P, C.c_int, C.c_int]),
     "wl_mixed_offsets_dev": (C.c_int, [_P, _P, _SZ, _P, _P]),
     "wl_mixed_decode_dev": (C.c_int, [_P, _P, _P, _P, _SZ, _P, _P, _P, _P, _P, _P]),
     "wl_mixed_set_offsets": (C.c_int, [_P, C.c_int, _P, C.c_int]),

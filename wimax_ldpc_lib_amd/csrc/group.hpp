@@ -23,10 +23,13 @@ struct GroupArgs {
     const uint32_t* buckets;           // [2 * 114] {first slot, count} per code, on the device
     unsigned long long* tile_counter;  // this group's tile counter (zeroed before the launch)
     unsigned enabled;                  // bit r: code (r, z) is decoded
+    int frames_per_wg;                 // F, 1 .. GroupShape<Z>::fmax
 };
 
-// The group's frames per workgroup: the most common tuned value over its
-// rates, ties to the smaller block.
+// The group's launch shape.  Frames per workgroup are a launch argument (the
+// tile body takes F at run time); the default is the value the mixed-batch
+// autotuner measured for this z (csrc/group_tuned.inc), else the most common
+// per-code tuned value over its rates, ties to the smaller block.
 template <int Z>
 struct GroupShape {
     // a rate's tuned F counts unless it was tuned to a wave-split shape (2
@@ -50,14 +53,23 @@ struct GroupShape {
         }
         return best > 0 ? best : 2;
     }
-    static constexpr int F = pick_f();
+    static constexpr int heuristic_f() { return pick_f(); }
+    static constexpr int F = group_tuned_f(Z) > 0 ? group_tuned_f(Z) : pick_f();
+    // largest F: kMaxThreads lanes and 64 KiB of LDS
+    static constexpr int max_f()
+    {
+        int f = kMaxThreads / Z;
+        while (f > 1 && kLdsPad + (size_t)f * lds_frame_stride<24 * Z, VAR_LEAN>() * sizeof(float) > (size_t)kMaxLdsBytes) f--;
+        return f;
+    }
+    static constexpr int fmax = max_f();
     // the register-lean shape for every rate: the hoisted-address shape
     // pushes the combined kernel past 168 VGPRs, and the two-halves shape
     // (six rates x one loop copy per wave) spills scalar registers and
     // measured slower here
     static constexpr int VAR = VAR_LEAN;
-    static constexpr int threads = ((F * Z + 63) / 64) * 64;
-    static constexpr size_t lds = kLdsPad + (size_t)F * lds_frame_stride<24 * Z, VAR>() * sizeof(float);
+    static constexpr int threads(int f) { return ((f * Z + 63) / 64) * 64; }
+    static constexpr size_t lds(int f) { return kLdsPad + (size_t)f * lds_frame_stride<24 * Z, VAR>() * sizeof(float); }
     template <int R>
     static constexpr int var()
     {
@@ -70,7 +82,7 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_group_kernel(GroupArgs g)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     using S = GroupShape<Z>;
-    constexpr int F = S::F;
+    const int F = g.frames_per_wg;
     constexpr int zi = index_of_z(Z);
     unsigned long long* slot = reinterpret_cast<unsigned long long*>(smem + kTileSlot);
     // Tiles of rate 0 first, then rate 1, ...: the counter only grows, so
@@ -118,29 +130,33 @@ __global__ __launch_bounds__(kMaxThreads) void tdmp_group_kernel(GroupArgs g)
 template <int Z>
 hipError_t launch_group(const GroupArgs& g, int grid, hipStream_t stream)
 {
-    hipLaunchKernelGGL(tdmp_group_kernel<Z>, dim3((unsigned)grid), dim3(GroupShape<Z>::threads), GroupShape<Z>::lds,
-                       stream, g);
+    const int f = g.frames_per_wg;
+    if (f < 1 || f > GroupShape<Z>::fmax) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(tdmp_group_kernel<Z>, dim3((unsigned)grid), dim3(GroupShape<Z>::threads(f)),
+                       GroupShape<Z>::lds(f), stream, g);
     return hipGetLastError();
 }
 
-// workgroups of the group kernel resident per CU (0 on error)
+// workgroups of the group kernel resident per CU at F frames per workgroup (0 on error)
 template <int Z>
-int occupancy_group()
+int occupancy_group(int f)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tdmp_group_kernel<Z>, GroupShape<Z>::threads,
-                                                     GroupShape<Z>::lds) != hipSuccess)
+    if (f < 1 || f > GroupShape<Z>::fmax) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tdmp_group_kernel<Z>, GroupShape<Z>::threads(f),
+                                                     GroupShape<Z>::lds(f)) != hipSuccess)
         return 0;
     return nb;
 }
 
 using GroupLaunch = hipError_t (*)(const GroupArgs&, int, hipStream_t);
-using GroupOccupancy = int (*)();
+using GroupOccupancy = int (*)(int);
 
 struct GroupEntry {
     GroupLaunch launch;
     GroupOccupancy occ;
-    int frames_per_wg;
+    int frames_per_wg;   // default (tuned) F
+    int fmax;
 };
 
 // defined in decode_g<k>.hip, one per lifting size
@@ -171,7 +187,7 @@ inline const GroupEntry* group_entry(int z)
     namespace wl {                                                                          \
     const GroupEntry* group_entry_z##Z()                                                    \
     {                                                                                       \
-        static const GroupEntry e{&launch_group<Z>, &occupancy_group<Z>, GroupShape<Z>::F}; \
+        static const GroupEntry e{&launch_group<Z>, &occupancy_group<Z>, GroupShape<Z>::F, GroupShape<Z>::fmax}; \
         return &e;                                                                          \
     }                                                                                       \
     }

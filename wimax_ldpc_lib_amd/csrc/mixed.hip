@@ -312,6 +312,7 @@ struct wl_mixed {
     int T[kNumRates] = {};
     int ncu = 0;
     int gocc[kNumZ] = {};   // resident workgroups per CU of each z group kernel
+    int gF[kNumZ] = {};     // frames per workgroup of each z group kernel
 };
 
 namespace {
@@ -393,7 +394,11 @@ int wl_mixed_create(int max_iter, unsigned flags, wl_mixed** out)
         m->enabled[c] = true;
         descs[c] = make_encode_desc_for(rate, z);
     }
-    for (int zi = 0; zi < kNumZ; zi++) m->gocc[zi] = group_entry(z_of_index(zi))->occ();
+    for (int zi = 0; zi < kNumZ; zi++) {
+        const GroupEntry* ge = group_entry(z_of_index(zi));
+        m->gF[zi] = ge->frames_per_wg;
+        m->gocc[zi] = ge->occ(m->gF[zi]);
+    }
     e = hipMalloc((void**)&m->counts, (kCodes + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&m->cursors, (kCodes + 1) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void**)&m->buckets, 2 * kCodes * sizeof(uint32_t));
@@ -456,9 +461,25 @@ int wl_mixed_get_launch(const wl_mixed* m, int rate, int z, int* frames_per_wg, 
     if (!m || !frames_per_wg || !variant || !grid) return mfail(WL_EINVAL, "null argument");
     if (rate < 0 || rate >= kNumRates || !valid_z(z)) return mfail(WL_EINVAL, "invalid rate/z");
     const int zi = index_of_z(z);
-    *frames_per_wg = group_entry(z)->frames_per_wg;
+    *frames_per_wg = m->gF[zi];
     *variant = 1;
     *grid = std::max(1, m->gocc[zi]) * m->num_cu;
+    return WL_OK;
+}
+
+int wl_mixed_set_group_frames_per_wg(wl_mixed* m, int z, int frames_per_wg)
+{
+    if (!m) return mfail(WL_EINVAL, "null mixed decoder");
+    if (!valid_z(z)) return mfail(WL_EINVAL, "invalid z");
+    const GroupEntry* ge = group_entry(z);
+    const int f = frames_per_wg == 0 ? ge->frames_per_wg : frames_per_wg;
+    if (f < 1 || f > ge->fmax) return mfail(WL_EINVAL, "frames_per_wg out of range for this z");
+    const int occ = ge->occ(f);
+    if (occ < 1) return mfail(WL_EINVAL, "no workgroup of this shape fits a CU");
+    const int zi = index_of_z(z);
+    m->gF[zi] = f;
+    m->gocc[zi] = occ;
+    drop_graph(m);
     return WL_OK;
 }
 
@@ -595,6 +616,7 @@ static int capture_decode(wl_mixed* m, int learned, int fallback_iters, const ui
             g.buckets = m->buckets;
             g.tile_counter = m->tiles + kCodes + j.zi;
             g.enabled = j.mask;
+            g.frames_per_wg = m->gF[j.zi];
             const int grid = std::max(1, m->gocc[j.zi]) * m->num_cu;
             e = group_entry(z_of_index(j.zi))->launch(g, grid, ss);
         } else {

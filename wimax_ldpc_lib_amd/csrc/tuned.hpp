@@ -32,4 +32,22 @@ constexpr int tuned_variant(int rate, int z)
     return -1;
 }
 
+// frames per workgroup of the mixed-batch group kernel of lifting size z,
+// measured by the autotuner on mixed batches (group_tuned.inc: {z, F}); 0
+// when the table has no entry
+struct GroupTuned {
+    int z, f;
+};
+
+constexpr GroupTuned kGroupTuned[] = {
+#include "group_tuned.inc"
+    {0, 0}};
+
+constexpr int group_tuned_f(int z)
+{
+    for (const GroupTuned& g : kGroupTuned)
+        if (g.z == z) return g.f;
+    return 0;
+}
+
 }  // namespace wl

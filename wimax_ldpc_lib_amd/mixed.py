@@ -42,6 +42,10 @@ class MixedDecoder:
         check(lib().wl_mixed_get_launch(self._h, int(rate), int(z), C.byref(f), C.byref(v), C.byref(g)))
         return f.value, v.value, g.value
 
+    def set_group_frames_per_wg(self, z, frames_per_wg):
+        """Frames per workgroup of the kernel of lifting size z (0: the tuned default)."""
+        check(lib().wl_mixed_set_group_frames_per_wg(self._h, int(z), int(frames_per_wg)))
+
     def offsets_dev(self, cfg, off, stream=None):
         check(lib().wl_mixed_offsets_dev(self._h, _ptr(cfg), cfg.numel(), _ptr(off), _stream(stream)))
 
