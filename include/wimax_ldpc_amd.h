@@ -115,6 +115,21 @@ int wl_decoder_get_frames_per_wg(const wl_decoder* dec, int* out);
 int wl_decoder_set_variant(wl_decoder* dec, int variant);
 int wl_decoder_get_variant(const wl_decoder* dec, int* out);
 
+/* Small-batch shape (no counterpart in the reference): contiguous
+ * normalised min-sum batches of at most max_frames frames are decoded one
+ * frame per workgroup with two lanes per parity row, which halves the serial
+ * work of a layer (batches that leave most of the GPU idle are bound by that
+ * chain) and, with early stop, retires each frame's workgroup as soon as the
+ * frame converges.  Results are identical to the other shapes.  0 = never;
+ * WL_SMALL_BATCH_AUTO = the default for the current device: with early stop
+ * up to 32 waves per SIMD (32768 frames at z <= 32 on 256 compute units,
+ * 10922 at z = 96), without it up to 3 (3072 and 1024).  Learned, offset
+ * and plain min-sum decoders and run-time codes do not use it (get reports
+ * 0 for them). */
+#define WL_SMALL_BATCH_AUTO ((size_t)-1)
+int wl_decoder_set_small_batch(wl_decoder* dec, size_t max_frames);
+int wl_decoder_get_small_batch(const wl_decoder* dec, size_t* max_frames);
+
 /* Autotuned launch shape compiled into the library for (rate, z): returns 0,
  * or 1 when the table has no entry and the heuristic default is reported. */
 int wl_tuned_launch(int rate, int z, int* frames_per_wg, int* variant);

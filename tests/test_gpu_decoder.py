@@ -49,8 +49,11 @@ def test_rate12_z24_1024_frames_early_stop(gpu, oracle_mod):
     llr, _ = noisy_llr(code, 1024, 2.5, seed=11)
     o = code.decode(llr, max_iter=10, early_stop=True)
     dec = W.ldpc_decoder(0, 24, 10, early_stop=True)
-    g = _gpu_decode(gpu, dec, llr)
-    _compare(g, o, "rate 1/2 z=24")
+    assert dec.small_batch >= 1024   # 256 compute units: this batch takes the small-batch kernel
+    for small in (None, 0):
+        dec.small_batch = small
+        g = _gpu_decode(gpu, dec, llr)
+        _compare(g, o, "rate 1/2 z=24 small_batch %r" % small)
     assert o["ret"].sum() >= 1 and len(np.unique(o["iters"])) > 3   # failures and varied stops
 
 
@@ -61,7 +64,9 @@ def test_rate56_z96_fixed10(gpu, oracle_mod):
     llr, _ = noisy_llr(code, 512, 4.0, seed=12)
     o = code.decode(llr, max_iter=10, early_stop=False)
     dec = W.ldpc_decoder(5, 96, 10, early_stop=False)
-    _compare(_gpu_decode(gpu, dec, llr), o, "rate 5/6 z=96")
+    for small in (0, 1 << 30):   # the row-per-lane kernel, the small-batch kernel
+        dec.small_batch = small
+        _compare(_gpu_decode(gpu, dec, llr), o, "rate 5/6 z=96 small_batch %d" % small)
 
 
 @pytest.mark.parametrize("rate", range(6))
@@ -80,7 +85,9 @@ def test_all_codes(gpu, oracle_mod, rate):
         for early in (True, False):
             o = code.decode(llr, max_iter=8, early_stop=early)
             dec = W.ldpc_decoder(rate, z, 8, early_stop=early)
-            _compare(_gpu_decode(gpu, dec, llr), o, "rate %d z %d early %d" % (rate, z, early))
+            for small in (None, 0):   # the small-batch kernel (default at 24 frames), the row-per-lane one
+                dec.small_batch = small
+                _compare(_gpu_decode(gpu, dec, llr), o, "rate %d z %d early %d small_batch %r" % (rate, z, early, small))
 
 
 @pytest.mark.parametrize("rate,z", [(0, 24), (5, 96), (3, 52), (1, 60)])
@@ -91,6 +98,7 @@ def test_frames_per_workgroup_and_ragged_batches(gpu, oracle_mod, rate, z):
     llr, _ = noisy_llr(code, 37, 2.0 if rate < 3 else 3.0, seed=7 * z + rate)
     o = code.decode(llr, max_iter=6, early_stop=True)
     dec = W.ldpc_decoder(rate, z, 6, early_stop=True)
+    dec.small_batch = 0   # the row-per-lane kernel at every batch size
     fmax = 1
     while True:
         try:
@@ -127,12 +135,14 @@ def test_edge_cases(gpu, oracle_mod):
         for early in (True, False):
             o = code.decode(llr, max_iter=max_iter, early_stop=early)
             dec = W.ldpc_decoder(5, 96, max_iter, early_stop=early)
-            g = _gpu_decode(gpu, dec, llr)
-            _compare(g, o, "edge max_iter %d early %d" % (max_iter, early))
-            if early:
-                assert (g["iters"][:4] == 0).all() and bits_equal_f32(g["post"][:4], clean)
-                if max_iter == 0:
-                    assert (g["ret"] == 0).all()   # lib/ldpc_decoder.cc:124,141 quirk
+            for small in (None, 0):
+                dec.small_batch = small
+                g = _gpu_decode(gpu, dec, llr)
+                _compare(g, o, "edge max_iter %d early %d small_batch %r" % (max_iter, early, small))
+                if early:
+                    assert (g["iters"][:4] == 0).all() and bits_equal_f32(g["post"][:4], clean)
+                    if max_iter == 0:
+                        assert (g["ret"] == 0).all()   # lib/ldpc_decoder.cc:124,141 quirk
 
 
 def test_host_api_semantics(gpu, oracle_mod):
@@ -213,6 +223,7 @@ def test_two_halves_code_shape(gpu, oracle_mod, rate, z):
     for early in (True, False):
         o = code.decode(llr, max_iter=9, early_stop=early)
         dec = W.ldpc_decoder(rate, z, 9, early_stop=early)
+        dec.small_batch = 0
         dec.variant = 0
         dec.frames_per_wg = 3
         with pytest.raises(W.WLError):
@@ -245,6 +256,7 @@ def test_quarters_code_shape(gpu, oracle_mod, rate, z):
     for early in (True, False):
         o = code.decode(llr, max_iter=9, early_stop=early)
         dec = W.ldpc_decoder(rate, z, 9, early_stop=early)
+        dec.small_batch = 0
         dec.variant = 0
         dec.frames_per_wg = 2
         with pytest.raises(W.WLError):
@@ -257,6 +269,50 @@ def test_quarters_code_shape(gpu, oracle_mod, rate, z):
         assert dec.variant == 1
     with pytest.raises(W.WLError):
         W.ldpc_decoder(5, 96, 5).variant = 3      # more than four waves: no such shape
+
+
+@pytest.mark.parametrize("rate,z", [(0, 24), (1, 60), (3, 32), (5, 96), (2, 92), (4, 36)])
+def test_small_batch_kernel_matches_row_per_lane_kernel(gpu, oracle_mod, rate, z):
+    """The small-batch kernel (one frame per workgroup, two lanes per row)
+    forced onto a 3000-frame batch near the waterfall equals the row-per-lane
+    kernel bit for bit in both stop modes (posteriors, hard decisions, status),
+    and a spread sample equals the oracle."""
+    import wimax_ldpc_lib_amd as W
+    code = oracle_mod.Code(rate, z)
+    F = 3000
+    if rate == 4:
+        llr = np.random.default_rng(z).normal(0.9, 2.0, (F, code.n)).astype(np.float32)
+    else:
+        llr, _ = noisy_llr(code, F, {0: 1.9, 1: 2.5, 2: 2.5, 3: 2.9, 5: 3.5}[rate], seed=900 + rate * 97 + z)
+    sample = np.arange(0, F, 149)
+    for early in (True, False):
+        dec = W.ldpc_decoder(rate, z, 10, early_stop=early)
+        dec.small_batch = 1 << 30
+        a = _gpu_decode(gpu, dec, llr)
+        dec.small_batch = 0
+        b = _gpu_decode(gpu, dec, llr)
+        for k in a:
+            assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), (k, early)
+        o = code.decode(np.ascontiguousarray(llr[sample]), 10, early)
+        _compare({k: v[sample] for k, v in a.items()}, o, "small-batch sample rate %d z %d early %d" % (rate, z, early))
+        if early and rate != 4:   # 3/4B: random LLRs, no frame converges
+            assert len(np.unique(a["iters"])) > 2
+
+
+def test_small_batch_setting(gpu):
+    """Default threshold per lifting size, 0 disables, None restores; the
+    offset / plain min-sum rules report 0 (they never take that kernel)."""
+    import wimax_ldpc_lib_amd as W
+    d24, d96 = W.ldpc_decoder(0, 24, 10), W.ldpc_decoder(5, 96, 10)
+    assert d24.small_batch > d96.small_batch > 0
+    assert W.ldpc_decoder(0, 24, 10, early_stop=False).small_batch < d24.small_batch
+    d24.small_batch = 0
+    assert d24.small_batch == 0
+    d24.small_batch = 77
+    assert d24.small_batch == 77
+    d24.small_batch = None
+    assert d24.small_batch == W.ldpc_decoder(0, 24, 10).small_batch
+    assert W.ldpc_decoder(0, 24, 10, rule="oms").small_batch == 0
 
 
 def test_nan_input_stays_in_its_frame(gpu, oracle_mod):
@@ -277,10 +333,12 @@ def test_nan_input_stays_in_its_frame(gpu, oracle_mod):
         for early in (True, False):
             o = code.decode(llr[keep], 10, early)
             dec = W.ldpc_decoder(rate, z, 10, early_stop=early)
-            g = _gpu_decode(gpu, dec, llr)
-            assert np.isnan(g["post"][bad, 17])
-            gk = {k: v[keep] for k, v in g.items()}
-            _compare(gk, o, "NaN batch rate %d z %d early %d" % (rate, z, early))
+            for small in (None, 0):
+                dec.small_batch = small
+                g = _gpu_decode(gpu, dec, llr)
+                assert np.isnan(g["post"][bad, 17])
+                gk = {k: v[keep] for k, v in g.items()}
+                _compare(gk, o, "NaN batch rate %d z %d early %d small_batch %r" % (rate, z, early, small))
 
 
 @pytest.mark.parametrize("rule", ["oms", "ms"])

@@ -15,6 +15,7 @@ WL_FLAG_EARLY_STOP = 0x1
 WL_FLAG_MIXED_UNORDERED = 0x2
 WL_FLAG_OMS = 0x4
 WL_FLAG_MS = 0x8
+WL_SMALL_BATCH_AUTO = C.c_size_t(-1).value
 
 # every function the header declares: name -> (restype, argtypes)
 _P = C.c_void_p
@@ -31,6 +32,8 @@ SIGNATURES = {
     "wl_decoder_get_frames_per_wg": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "wl_decoder_set_variant": (C.c_int, [_P, C.c_int]),
     "wl_decoder_get_variant": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "wl_decoder_set_small_batch": (C.c_int, [_P, _SZ]),
+    "wl_decoder_get_small_batch": (C.c_int, [_P, C.POINTER(_SZ)]),
     "wl_tuned_launch": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "wl_decode_f32_dev": (C.c_int, [_P, _P, _SZ, _P, _P, _P, _P, _P, _P]),
     "wl_decode_f32_host": (C.c_int, [_P, _P, _SZ, _P, _P, _P]),

@@ -175,6 +175,19 @@ class ldpc_decoder(ldpc):
     def variant(self, v):
         check(lib().wl_decoder_set_variant(self._h, int(v)))
 
+    @property
+    def small_batch(self):
+        """Contiguous batches of at most this many frames use the small-batch
+        kernel (one frame per workgroup, two lanes per row); 0 = never."""
+        v = C.c_size_t()
+        check(lib().wl_decoder_get_small_batch(self._h, C.byref(v)))
+        return v.value
+
+    @small_batch.setter
+    def small_batch(self, n):
+        """None: the default for this device; 0: never."""
+        check(lib().wl_decoder_set_small_batch(self._h, capi.WL_SMALL_BATCH_AUTO if n is None else int(n)))
+
     def set_offsets(self, offsets):
         """Learned offsets [T][mb][c] (numpy float32), or None for plain NMS."""
         if offsets is None:

@@ -17,6 +17,7 @@
 #include "errors.hpp"
 #include "generic.hpp"
 #include "qc_host.hpp"
+#include "tdmp_pair.hpp"
 #include "tuned.hpp"
 
 #define WL_VERSION_STR "wimax_ldpc_amd 0.1 (gfx950)"
@@ -168,6 +169,9 @@ struct wl_decoder {
     int frames_per_wg = 0;
     int variant = 0;
     const wl::DecodeEntry* entry = nullptr;
+    // small-batch kernel (contiguous NMS batches of at most small_batch frames)
+    wl::PairLaunch pair = nullptr;
+    size_t small_batch = 0;
     DevBuf<float> offsets;
     int T = 0;
     // host-call staging
@@ -253,6 +257,19 @@ int wl_code_table(int rate, int z, int16_t* out, size_t out_len)
     return WL_OK;
 }
 
+// the small-batch threshold for the current device (256 compute units where
+// none can be queried)
+static size_t default_small_batch(int z, bool early_stop)
+{
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) {
+        (void)hipGetLastError();
+        ncu = 256;
+    }
+    return wl::pair_default_max_frames(z, ncu, early_stop);
+}
+
 int wl_decoder_create(int rate, int z, int max_iter, unsigned flags, wl_decoder** out)
 {
     if (!out) return fail(WL_EINVAL, "null output");
@@ -278,7 +295,30 @@ int wl_decoder_create(int rate, int z, int max_iter, unsigned flags, wl_decoder*
         d->frames_per_wg = default_frames_per_wg(z, d->entry->fmax);
         d->variant = 0;
     }
+    d->pair = wl::pair_launcher(rate, z);
+    d->small_batch = default_small_batch(z, (flags & WL_FLAG_EARLY_STOP) != 0);
     *out = d;
+    return WL_OK;
+}
+
+int wl_decoder_set_small_batch(wl_decoder* dec, size_t max_frames)
+{
+    if (!dec) return fail(WL_EINVAL, "null decoder");
+    if (dec->generic || !dec->pair) return max_frames == 0 || max_frames == WL_SMALL_BATCH_AUTO
+                                               ? WL_OK
+                                               : fail(WL_ENOTSUP, "run-time codes have no small-batch kernel");
+    dec->small_batch = max_frames == WL_SMALL_BATCH_AUTO
+                           ? default_small_batch(dec->code.info.z, (dec->flags & WL_FLAG_EARLY_STOP) != 0)
+                           : max_frames;
+    return WL_OK;
+}
+
+int wl_decoder_get_small_batch(const wl_decoder* dec, size_t* max_frames)
+{
+    if (!dec || !max_frames) return fail(WL_EINVAL, "null argument");
+    // offset / plain min-sum and learned decoders never take the small-batch kernel
+    const bool nms = !(dec->flags & (WL_FLAG_OMS | WL_FLAG_MS)) && dec->T == 0;
+    *max_frames = dec->pair && nms ? dec->small_batch : 0;
     return WL_OK;
 }
 
@@ -422,6 +462,8 @@ static int decode_launch(wl_decoder* dec, bool learned, int max_iter, bool early
     } else if (!force_nms && (dec->flags & (WL_FLAG_OMS | WL_FLAG_MS))) {
         a.rule = (dec->flags & WL_FLAG_OMS) ? 0 : 1;
         e = dec->entry->alt(a, s);
+    } else if (dec->pair && !idx && !bucket && nframes <= dec->small_batch) {
+        e = dec->pair(a, s);
     } else {
         e = dec->entry->nms[dec->variant](a, s);
     }

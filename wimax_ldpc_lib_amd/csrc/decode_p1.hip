@@ -1,0 +1,3 @@
+// decode_p1.hip -- small-batch TDMP decoder kernels (two lanes per row) for coderate 1 at all 19 lifting factors.
+#include "tdmp_pair.hpp"
+WL_DEFINE_PAIR_TABLE(1)
