@@ -143,7 +143,11 @@ int wl_tuned_launch(int rate, int z, int* frames_per_wg, int* variant);
  *   ret      [nframes]     decode()'s return value: 0 syndrome clean, 1 failed, nullable
  *   iters    [nframes]     iterations run (0 if the input already satisfies H), nullable
  *   synd     [nframes]     failing parity checks of the output hard decisions
- *                          (compute_syndrome(hd, false), lib/ldpc_decoder.cc:67-104), nullable */
+ *                          (compute_syndrome(hd, false), lib/ldpc_decoder.cc:67-104), nullable
+ * NaN LLRs are outside the bit-exact contract: the reference is compiled with
+ * -ffast-math (lib/CMakeLists.txt:6), which leaves its comparisons with NaN
+ * undefined.  A frame holding a NaN decodes to some result (its NaN
+ * posteriors stay NaN); every other frame of the batch is unaffected. */
 int wl_decode_f32_dev(wl_decoder* dec, const float* llr, size_t nframes, float* post,
                       uint8_t* hard, uint8_t* ret, uint8_t* iters, uint16_t* synd, void* stream);
 
