@@ -317,6 +317,12 @@ def encode(args, np, torch, W):
     bytes_per_frame = k + n
     achieved = F * bytes_per_frame / (kavg * 1e-3) / 1e9
     log("encode: kernel avg %.3f ms, step %.3f ms, %.1f coded Gbit/s, %.0f GB/s" % (kavg, ms_step, value, achieved))
+    # HBM bytes of this kernel from the committed rocprofv3 counter runs
+    # (FETCH_SIZE x2 and WRITE_SIZE x1 after copy calibration)
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "encode_r12z96_f524288_traffic.json")
+    if os.path.exists(pmc):
+        traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -338,7 +344,7 @@ def encode(args, np, torch, W):
            "config": {"workload": "dual-diagonal encoder, rate 1/2, z=96, n=2304, k=1152, %d frames, one byte per "
                                   "bit in and out" % F, "frames": F},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                        "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel_ms": kavg,
+                        "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "kernel_ms": kavg,
                         "bytes_per_frame": bytes_per_frame},
            "cpu_baseline": cpu,
            "vs_baseline_note": "the reference's README figure, 171.904706 coded Mbit/s, i7-4771 (4 threads)"}

@@ -37,6 +37,35 @@ def test_encoder_matches_oracle(gpu, oracle_mod, rate):
         assert np.array_equal(host, want)
 
 
+@pytest.mark.parametrize("rate, z", [(0, 96), (3, 28), (5, 24), (4, 52), (1, 64), (2, 36)])
+def test_encoder_tiles_and_pointer_alignment(gpu, oracle_mod, rate, z):
+    """Compiled codes are encoded 64 frames per workgroup as bit-sliced words
+    with 16-byte accesses (k % 16 == 0) or 8-byte ones (3/4 at z % 8 == 4);
+    pointers not aligned to that take the byte-per-frame kernel.  Three full
+    tiles and a ragged one, bytes of any value (nonzero means 1, copied as
+    given), at pointer offsets 0, 8 and 1: every frame equals the oracle and
+    no byte outside the output is written."""
+    import wimax_ldpc_lib_amd as W
+    torch = gpu
+    code = oracle_mod.Code(rate, z)
+    F = 3 * 64 + 17
+    rng = np.random.default_rng(7000 + rate * 100 + z)
+    info = rng.integers(0, 2, (F, code.k), dtype=np.uint8)
+    wild = rng.random((F, code.k)) < 0.1
+    info[wild] = rng.integers(0, 256, int(wild.sum()), dtype=np.uint8)
+    want = code.encode(info)
+    enc = W.ldpc_encoder(rate, z)
+    for off in (0, 8, 1):
+        buf_in = torch.zeros(F * code.k + 64, dtype=torch.uint8, device="cuda")
+        buf_in[off:off + F * code.k] = torch.from_numpy(info.ravel()).cuda()
+        buf_out = torch.full((F * code.n + 64,), 0xAB, dtype=torch.uint8, device="cuda")
+        enc.encode_dev(buf_in[off:off + F * code.k], buf_out[off:off + F * code.n])
+        torch.cuda.synchronize()
+        out = buf_out.cpu().numpy()
+        assert np.array_equal(out[off:off + F * code.n].reshape(F, code.n), want), (rate, z, off)
+        assert (out[:off] == 0xAB).all() and (out[off + F * code.n:] == 0xAB).all(), (rate, z, off)
+
+
 def test_encoder_matches_reference_fixtures(gpu):
     import wimax_ldpc_lib_amd as W
     fx = np.load(os.path.join(GOLDEN, "encoder_ref.npz"))

@@ -60,14 +60,17 @@ __device__ __forceinline__ void encode_lambda_ct(const uint8_t* u, int t, bool a
 }
 
 // Parity bits of the frame from its lambda blocks, lane t < z; P_0 scratch at
-// p0 (z bytes).  Writes bit i of the parity part through put(i, bit),
-// i in [0, m).  All threads of the workgroup call it (one barrier).
-template <class Put>
-__device__ __forceinline__ void encode_parity(const EncodeDesc& d, const uint8_t (&V)[kMaxLayers], uint8_t* p0,
+// p0 (z words).  Writes bit i of the parity part through put(i, bit),
+// i in [0, m).  All threads of the workgroup call it (one barrier).  T is
+// uint8_t (one frame, a 0/1 byte per bit) or a bit-sliced word (bit f of
+// every word belongs to frame f of a tile): the steps are XORs only, so
+// each bit position of T runs the same encoder.
+template <class T, class Put>
+__device__ __forceinline__ void encode_parity(const EncodeDesc& d, const T (&V)[kMaxLayers], T* p0,
                                               int t, bool active, Put&& put)
 {
     const int z = d.z;
-    uint8_t P[kMaxLayers];
+    T P[kMaxLayers];
 #pragma unroll
     for (int q = 0; q < kMaxLayers; q++) P[q] = 0;
     const int mb = d.mb, X = d.x;
@@ -77,7 +80,7 @@ __device__ __forceinline__ void encode_parity(const EncodeDesc& d, const uint8_t
         // lambda_q, so p0[t] = S[(t - h_mid) mod z].  Then row 0 gives
         // p_1 = lambda_0 + P^h_top p0 and row q (1 <= q <= mb-2)
         // p_{q+1} = lambda_q + p_q (+ P^h_mid p0 = S at q = x).
-        uint8_t S = 0;
+        T S = 0;
 #pragma unroll
         for (int q = 0; q < kMaxLayers; q++)
             if (q < mb) S ^= V[q];
@@ -90,12 +93,12 @@ __device__ __forceinline__ void encode_parity(const EncodeDesc& d, const uint8_t
             a1 += a1 < 0 ? z : 0;
             a1 -= a1 >= z ? z : 0;
             put(t, p0[a0]);
-            uint8_t p = V[0] ^ p0[a1];
+            T p = V[0] ^ p0[a1];
             put(z + t, p);
 #pragma unroll
             for (int q = 1; q < kMaxLayers - 1; q++)
                 if (q <= mb - 2) {
-                    p = V[q] ^ p ^ (q == X ? S : (uint8_t)0);
+                    p = V[q] ^ p ^ (q == X ? S : (T)0);
                     put((q + 1) * z + t, p);
                 }
         }
@@ -110,23 +113,23 @@ __device__ __forceinline__ void encode_parity(const EncodeDesc& d, const uint8_t
 #pragma unroll
     for (int q = kMaxLayers - 2; q >= 1; q--)
         if (q <= mb - 2 && q >= X + 1) P[q] = V[q] ^ P[q + 1];
-    uint8_t vx = 0, px1 = 0, px = 0;
+    T vx = 0, px1 = 0, px = 0;
 #pragma unroll
     for (int q = 0; q < kMaxLayers; q++) {
         if (q == X) { vx = V[q]; px = P[q]; }
         if (q == X + 1) px1 = P[q];
     }
-    const uint8_t pz = vx ^ px1 ^ px;
+    const T pz = vx ^ px1 ^ px;
     if (active) p0[t] = pz;
     __syncthreads();
     if (active) {
         int tr = t + d.d;
         tr %= z;
-        const uint8_t corr = p0[tr];
+        const T corr = p0[tr];
         put(t, pz);
 #pragma unroll
         for (int q = 1; q < kMaxLayers; q++)
-            if (q < mb) put(q * z + t, (uint8_t)(P[q] ^ corr));
+            if (q < mb) put(q * z + t, (T)(P[q] ^ corr));
     }
 }
 

@@ -104,6 +104,215 @@ __global__ __launch_bounds__(512) void encode_kernel(EncodeDesc d, int F, int pa
     else store_parity<1>(par, cw, k, n, m, Feff, frame0);
 }
 
+// Bit-sliced encoder of the compiled codes.  A workgroup encodes a tile of
+// 64 frames at once: bit f of every 64-bit word belongs to frame f of the
+// tile, so the XOR steps of the encoder (lambda blocks, prediction and
+// correction, back-substitution) run once for all 64 frames.
+//   1. The tile's info bytes come in chunks of CB bytes per frame: coalesced
+//      VEC-byte loads, copied to the codeword's systematic part (the info
+//      bytes as given, lib/ldpc_encoder.cc:90) and staged in LDS.  Lane f of
+//      a wave reads byte j of frame f, and one ballot of (byte != 0) is the
+//      word of position j (the reference's `1 && infoword[...]`, :43),
+//      stored to LDS by lane 0.
+//   2. Lane t < z computes lambda_q[t] of every base row from the words
+//      (:29-48) and the parity words (:55-103), as encode_parity does for one
+//      frame.
+//   3. Lane (fs, q) of a wave holds the parity words of VEC positions and
+//      writes, for frames 4g + fs, the 0/1 bytes of its frame's bits as one
+//      VEC-byte store: 16 lanes store 16 VEC consecutive bytes of one frame,
+//      whole cache lines (4-byte stores, two half lines per line, measured
+//      12 % more HBM write traffic).
+// This trades the per-frame, per-position work of encode_kernel (one byte
+// per info slot, position and frame) for two byte<->bit transposes of about
+// 2 and 1.75 VALU operations per 64 frame-bits.
+constexpr int kBsFrames = 64;
+constexpr int kBsThreads = 256;
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+
+// bytes per frame staged per step: the largest multiple of VEC that divides
+// k and is at most 128 (four waves of at most 32 positions each)
+template <int K, int VEC>
+constexpr int bs_chunk()
+{
+    int c = 128 - 128 % VEC;
+    while (c > VEC && K % c != 0) c -= VEC;
+    return c;
+}
+
+// staging row stride: a multiple of 4 bytes, an odd number of dwords, so the
+// 64 lanes of a byte read (one row each) hit 64 different banks
+template <int CB>
+constexpr int bs_row()
+{
+    return CB + 4;
+}
+
+// access width of the compiled code (rate, z): 16 bytes where k % 16 == 0,
+// else 8 (k = kb z with z % 4 == 0 is always a multiple of 8)
+template <int RATE, int Z>
+constexpr int bs_vec()
+{
+    return Code<RATE, Z>::K % 16 == 0 ? 16 : 8;
+}
+
+// LDS of a tile: info words [K] and P_0 [Z], then one region holding the
+// staged bytes (phase 1) and later the parity words [M] (phases 2 and 3)
+template <int RATE, int Z>
+constexpr size_t bs_lds()
+{
+    using CD = Code<RATE, Z>;
+    constexpr int CB = bs_chunk<CD::K, bs_vec<RATE, Z>()>();
+    constexpr size_t stg = (size_t)kBsFrames * bs_row<CB>(), par = (size_t)CD::M * sizeof(uint64_t);
+    return (size_t)(CD::K + Z) * sizeof(uint64_t) + (stg > par ? stg : par);
+}
+
+template <int RATE, int Z>
+__global__ __launch_bounds__(kBsThreads) void encode_bs_kernel(EncodeDesc d, const uint8_t* __restrict__ info,
+                                                               uint8_t* __restrict__ cw, unsigned long long nframes)
+{
+    using CD = Code<RATE, Z>;
+    constexpr int K = CD::K, M = CD::M, N = CD::N, KB = 24 - CD::NL;
+    constexpr int VEC = bs_vec<RATE, Z>();
+    constexpr int CB = bs_chunk<K, VEC>();
+    constexpr int SROW = bs_row<CB>();
+    constexpr int WPF = CB / VEC;              // loads per frame and chunk
+    constexpr int PPW = CB / 4;                // positions per wave and chunk
+    static_assert(K % CB == 0 && CB % VEC == 0 && CB % 4 == 0 && PPW <= 32, "chunk shape");
+    static_assert(M % 4 == 0 && N % 16 == 0, "parity quads and codeword rows");
+    using Word = typename std::conditional<VEC == 16, u32x4, u32x2>::type;
+
+    extern __shared__ __align__(16) uint8_t esm[];
+    uint64_t* wl = reinterpret_cast<uint64_t*>(esm);    // [K] info words
+    uint64_t* p0 = wl + K;                               // [Z] P_0 scratch
+    uint8_t* stg = reinterpret_cast<uint8_t*>(p0 + Z);   // [64][SROW] staged bytes (phase 1)
+    uint64_t* pl = reinterpret_cast<uint64_t*>(stg);     // [M] parity words (phases 2, 3)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const unsigned long long frame0 = (unsigned long long)blockIdx.x * kBsFrames;
+    const unsigned long long left = nframes - frame0;
+    const int fe = left < (unsigned long long)kBsFrames ? (int)left : kBsFrames;
+
+    // 1. info bytes -> systematic part + info words.  The loads of the next
+    // chunk are issued before the current chunk is transposed, so they are in
+    // flight across its barriers.  (Rows of frames >= fe are left as they
+    // are: their bits never leave the tile.)
+    constexpr int LPT = (kBsFrames * WPF + kBsThreads - 1) / kBsThreads;   // loads per thread and chunk
+    Word v[LPT];
+    auto load = [&](int j0) {
+#pragma unroll
+        for (int u = 0; u < LPT; u++) {
+            const int i = tid + u * kBsThreads, f = i / WPF, w = i - f * WPF;
+            if (i < kBsFrames * WPF && f < fe)
+                v[u] = __builtin_nontemporal_load(reinterpret_cast<const Word*>(info + (frame0 + f) * K + j0) + w);
+        }
+    };
+    load(0);
+    for (int j0 = 0; j0 < K; j0 += CB) {
+#pragma unroll
+        for (int u = 0; u < LPT; u++) {
+            const int i = tid + u * kBsThreads, f = i / WPF, w = i - f * WPF;
+            if (i < kBsFrames * WPF && f < fe) {
+                __builtin_nontemporal_store(v[u], reinterpret_cast<Word*>(cw + (frame0 + f) * N + j0) + w);
+                uint32_t* s = reinterpret_cast<uint32_t*>(stg + f * SROW + VEC * w);
+                s[0] = v[u].x;
+                s[1] = v[u].y;
+                if constexpr (VEC == 16) {
+                    s[2] = v[u].z;
+                    s[3] = v[u].w;
+                }
+            }
+        }
+        __syncthreads();
+        if (j0 + CB < K) load(j0 + CB);
+        // all of the lane's byte reads first, then the ballots in groups of
+        // eight, written by lane 0 (one exec switch per group)
+        uint32_t b[PPW];
+#pragma unroll
+        for (int p = 0; p < PPW; p++) b[p] = stg[lane * SROW + wave * PPW + p];
+#pragma unroll
+        for (int g0 = 0; g0 < PPW; g0 += 8) {
+            uint64_t m[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (g0 + u < PPW) m[u] = __builtin_amdgcn_ballot_w64(b[g0 + u] != 0);
+            if (lane == 0) {
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (g0 + u < PPW) wl[j0 + wave * PPW + g0 + u] = m[u];
+            }
+        }
+        __syncthreads();
+    }
+
+    // 2. lambda and parity words, lane t < z
+    const bool act = tid < Z;
+    const int t = act ? tid : 0;
+    uint64_t V[kMaxLayers];
+#pragma unroll
+    for (int q = 0; q < kMaxLayers; q++) V[q] = 0;
+    if (act) {
+        static_for<0, CD::NL>([&](auto Lc) {
+            constexpr int L = decltype(Lc)::value;
+            uint64_t v = 0;
+            static_for<0, CD::deg(L)>([&](auto Jc) {
+                constexpr int J = decltype(Jc)::value;
+                constexpr int b = CD::blk(L, J), s = CD::sh(L, J);
+                if constexpr (b < KB) {
+                    const int p = t >= Z - s ? t + s - Z : t + s;
+                    v ^= wl[b * Z + p];
+                }
+            });
+            V[L] = v;
+        });
+    }
+    encode_parity(d, V, p0, t, act, [&](int i, uint64_t w) { pl[i] = w; });
+    __syncthreads();
+
+    // 3. parity words -> 0/1 bytes of the parity part.  Lane (fs, q) holds
+    // the words of positions i0 .. i0 + VEC - 1 of a block of 16 VEC
+    // positions and writes them for frames 4 g + fs as one VEC-byte store, so
+    // 16 lanes write 16 VEC consecutive bytes of a frame (whole cache lines).
+    const int fs = lane >> 4, q = lane & 15;
+    constexpr int PB = 16 * VEC;
+    static_assert(M % VEC == 0, "parity part in whole lanes");
+    for (int i0 = VEC * q; i0 < (M + PB - 1) / PB * PB; i0 += PB) {
+        const bool qa = i0 < M;
+        uint32_t lo[VEC], hi[VEC];
+#pragma unroll
+        for (int e = 0; e < VEC; e += 2) {
+            u32x4 x = {0, 0, 0, 0};
+            if (qa) x = *reinterpret_cast<const u32x4*>(pl + i0 + e);
+            lo[e] = x.x;
+            hi[e] = x.y;
+            lo[e + 1] = x.z;
+            hi[e + 1] = x.w;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int f = 4 * (wave + 4 * u) + fs;   // frames 0..31 for u < 2, 32..63 after
+            const uint32_t* w = u < 2 ? lo : hi;
+            const uint32_t sh = (uint32_t)(f & 31);
+            uint32_t o[VEC / 4];
+#pragma unroll
+            for (int c = 0; c < VEC / 4; c++) {
+                uint32_t x = __builtin_amdgcn_ubfe(w[4 * c], sh, 1);
+                x |= __builtin_amdgcn_ubfe(w[4 * c + 1], sh, 1) << 8;
+                x |= __builtin_amdgcn_ubfe(w[4 * c + 2], sh, 1) << 16;
+                x |= __builtin_amdgcn_ubfe(w[4 * c + 3], sh, 1) << 24;
+                o[c] = x;
+            }
+            if (qa && f < fe) {
+                Word out;
+                if constexpr (VEC == 16) out = Word{o[0], o[1], o[2], o[3]};
+                else out = Word{o[0], o[1]};
+                __builtin_nontemporal_store(out, reinterpret_cast<Word*>(cw + (frame0 + f) * N + K + i0));
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void channel_kernel(EncodeDesc d, unsigned long long seed,
                                                       unsigned long long frame_offset,
                                                       unsigned long long nframes, float eb_lo,
@@ -222,10 +431,39 @@ constexpr auto make_encode_kernels(std::integer_sequence<int, I...>)
 constexpr auto kEncodeKernelArr = make_encode_kernels(std::make_integer_sequence<int, kNumRates * kNumZ>{});
 constexpr const EncodeKernel* kEncodeKernels = kEncodeKernelArr.k;
 
+using EncodeBsKernel = void (*)(EncodeDesc, const uint8_t*, uint8_t*, unsigned long long);
+struct EncodeBsEntry {
+    EncodeBsKernel kern;
+    int vec;      // alignment the info and codeword pointers need
+    size_t lds;
+};
+
+template <int... I>
+constexpr auto make_encode_bs_kernels(std::integer_sequence<int, I...>)
+{
+    struct Arr { EncodeBsEntry e[sizeof...(I)]; };
+    return Arr{{EncodeBsEntry{&encode_bs_kernel<I / kNumZ, z_of_index(I % kNumZ)>,
+                              bs_vec<I / kNumZ, z_of_index(I % kNumZ)>(),
+                              bs_lds<I / kNumZ, z_of_index(I % kNumZ)>()}...}};
+}
+// [rate * 19 + z index]
+constexpr auto kEncodeBsArr = make_encode_bs_kernels(std::make_integer_sequence<int, kNumRates * kNumZ>{});
+
 hipError_t launch_encode(const EncodeDesc& d, const uint8_t* info, uint8_t* cw,
                          unsigned long long nframes, hipStream_t s)
 {
     if (nframes == 0) return hipSuccess;
+    const uintptr_t al = (uintptr_t)info | (uintptr_t)cw;
+    // compiled codes: the bit-sliced kernel, 64 frames per workgroup
+    if (d.compiled && d.rate >= 0 && d.rate < kNumRates && valid_z(d.z)) {
+        const EncodeBsEntry& e = kEncodeBsArr.e[d.rate * kNumZ + index_of_z(d.z)];
+        if (al % (uintptr_t)e.vec == 0) {
+            const unsigned long long grid = (nframes + kBsFrames - 1) / kBsFrames;
+            if (grid > 0x7fffffffull) return hipErrorInvalidValue;
+            hipLaunchKernelGGL(e.kern, dim3((unsigned)grid), dim3(kBsThreads), e.lds, s, d, info, cw, nframes);
+            return hipGetLastError();
+        }
+    }
     // F frames per pass (F z <= 512 lanes), passes per tile while the tile's
     // LDS (n + z bytes per frame) stays within 48 KiB
     int F = 512 / d.z;
@@ -238,7 +476,6 @@ hipError_t launch_encode(const EncodeDesc& d, const uint8_t* info, uint8_t* cw,
     const unsigned long long grid = (nframes + T - 1) / T;
     if (grid > 0x7fffffffull) return hipErrorInvalidValue;
     const size_t lds = (size_t)T * per;
-    const uintptr_t al = (uintptr_t)info | (uintptr_t)cw;
     int vec = 1;
     if (d.k % 16 == 0 && d.n % 16 == 0 && d.m % 16 == 0 && al % 16 == 0) vec = 16;
     else if (d.k % 4 == 0 && d.n % 4 == 0 && d.m % 4 == 0 && al % 4 == 0) vec = 4;
