@@ -531,39 +531,63 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         }
     };
 
-    // Failing parity checks among this lane's rows (lib/ldpc_decoder.cc:67-104
-    // over hd = rx >= 0), addressed like the layers (one select per wrapped
-    // slot).  Each slot's compare is ballotted into a wave-wide mask and the
-    // masks are XORed in scalar registers in slot order; the barrier on the
-    // running mask keeps each read next to its compare, so the check adds no
-    // live registers beside the messages.
+    // Whether this lane's row of layer L fails its parity check
+    // (lib/ldpc_decoder.cc:67-104 over hd = rx >= 0), addressed like the
+    // layers (one select per wrapped slot).  Each slot's compare is ballotted
+    // into a wave-wide mask and the masks are XORed in scalar registers in
+    // slot order; the barrier on the running mask keeps each read next to its
+    // compare, so the check adds no live registers beside the messages.
+    auto row_fails = [&](auto Lc, auto Wc) -> bool {
+        constexpr int L = decltype(Lc)::value;
+        int rr = r;
+        if constexpr (uses_rr) asm volatile("" : "+v"(rr));
+        const uint32_t ob0 = lds_addr(smem) + (uint32_t)(kLdsPad + ((active ? f : 0) * NS + r) * 4);
+        const uint32_t obW = ob0 - (uint32_t)(Z * 4);
+        cmask_t* mt = (cmask_t*)&WrapMasks<RATE, Z>::table.m[wphase][L][0];
+        if constexpr ((VAR & VAR_SMASK) != 0) asm volatile("" : "+s"(mt));
+        uint64_t pm = 0;
+        static_for<0, CD::deg(L)>([&](auto Jc) {
+            constexpr int J = decltype(Jc)::value;
+            float xv;
+            if constexpr ((VAR & VAR_SEL) != 0 && ((VAR & VAR_SMASK) != 0 || SPLIT)) {
+                xv = *lds_ptr(sel_addr(Lc, Jc, Wc, ob0, obW, mt));
+            } else if constexpr ((VAR & VAR_NOHOIST) != 0) {
+                xv = *at_r(Lc, Jc, rr);
+            } else {
+                xv = *at(Lc, Jc);
+            }
+            pm ^= __builtin_amdgcn_ballot_w64(xv >= 0.0f);
+            if constexpr ((J & 3) == 3) asm volatile("" : "+s"(pm));
+        });
+        return __builtin_amdgcn_inverse_ballot_w64(pm);
+    };
+    // Failing parity checks among this lane's rows: the syndrome weight.
     auto fail_rows = [&](auto Wc) -> int {
         int fails = 0;
+        static_for<0, NL>([&](auto Lc) { fails += row_fails(Lc, Wc) ? 1 : 0; });
+        return fails;
+    };
+    // The stop decisions only need to know whether some row of a frame fails
+    // (compute_syndrome(hd, true) returns at the first failing row,
+    // lib/ldpc_decoder.cc:67-104).  Lanes with `live` set cnt[f] = 1 as soon
+    // as their row of a layer fails; after layers 0, 1, 3 and 7 the wave
+    // re-reads the flags of its frames and checks no further layers once every
+    // live frame it holds rows of is known to fail.  A wave's own flag writes
+    // are visible to its later reads (LDS executes a wave's accesses in
+    // order); other waves' may or may not be yet, which only decides how soon
+    // it stops, not the result: the flags are read for the decision after a
+    // barrier.  Frames close to convergence are checked in full; frames with
+    // many errors fail in the first layer.
+    auto mark_fails = [&](auto Wc, bool live) {
+        bool need = __builtin_amdgcn_ballot_w64(live) != 0;
         static_for<0, NL>([&](auto Lc) {
             constexpr int L = decltype(Lc)::value;
-            int rr = r;
-            if constexpr (uses_rr) asm volatile("" : "+v"(rr));
-            const uint32_t ob0 = lds_addr(smem) + (uint32_t)(kLdsPad + ((active ? f : 0) * NS + r) * 4);
-            const uint32_t obW = ob0 - (uint32_t)(Z * 4);
-            cmask_t* mt = (cmask_t*)&WrapMasks<RATE, Z>::table.m[wphase][L][0];
-            if constexpr ((VAR & VAR_SMASK) != 0) asm volatile("" : "+s"(mt));
-            uint64_t pm = 0;
-            static_for<0, CD::deg(L)>([&](auto Jc) {
-                constexpr int J = decltype(Jc)::value;
-                float xv;
-                if constexpr ((VAR & VAR_SEL) != 0 && ((VAR & VAR_SMASK) != 0 || SPLIT)) {
-                    xv = *lds_ptr(sel_addr(Lc, Jc, Wc, ob0, obW, mt));
-                } else if constexpr ((VAR & VAR_NOHOIST) != 0) {
-                    xv = *at_r(Lc, Jc, rr);
-                } else {
-                    xv = *at(Lc, Jc);
-                }
-                pm ^= __builtin_amdgcn_ballot_w64(xv >= 0.0f);
-                if constexpr ((J & 3) == 3) asm volatile("" : "+s"(pm));
-            });
-            fails += __builtin_amdgcn_inverse_ballot_w64(pm) ? 1 : 0;
+            if (need) {
+                if (row_fails(Lc, Wc) && live) cnt[f] = 1;
+                if constexpr (L + 1 < NL && (L == 0 || L == 1 || L == 3 || L == 7))
+                    need = __builtin_amdgcn_ballot_w64(live && cnt[f] == 0) != 0;
+            }
         });
-        return fails;
     };
 
     bool done = !active;
@@ -574,7 +598,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
     auto run = [&](auto Wc) {
         if (a.early_stop) {
             // initial hard decision + syndrome (lib/ldpc_decoder.cc:130-139)
-            if (active && fail_rows(Wc)) cnt[f] = 1;
+            mark_fails(Wc, active);
             __syncthreads();
             if (active && cnt[f] == 0) done = true;
             __syncthreads();
@@ -588,7 +612,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             });
             if (a.early_stop) {
                 // lib/ldpc_decoder.cc:242-251
-                if (!done && fail_rows(Wc)) cnt[f] = 1;
+                mark_fails(Wc, !done);
                 __syncthreads();
                 if (!done && cnt[f] == 0) {
                     done = true;
@@ -599,11 +623,17 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             }
         }
         __syncthreads();
-        if (a.synd || a.ret) {
-            if (active) {
+        // Final syndrome: the weight where it is asked for; with fixed
+        // iterations the return value only needs to know whether a row
+        // fails; with early stop it follows from the stop decisions.  Frames
+        // that stopped early satisfy every check (weight 0, cnt[f] stays 0).
+        if (a.synd) {
+            if (!done) {
                 const int fr = fail_rows(Wc);
                 if (fr) atomicAdd(&cnt[f], (uint32_t)fr);
             }
+        } else if (a.ret && !a.early_stop) {
+            mark_fails(Wc, !done);
         }
     };
     if constexpr (SPLIT) {
