@@ -55,6 +55,25 @@ __global__ __launch_bounds__(kGenMaxZ) void generic_kernel(GenArgs a)
         return fails;
     };
 
+    // whether some row of the frame fails, for the stop decisions (compute_
+    // syndrome(hd, true) returns at the first failing row): a lane sets
+    // cnt[0] at its first failing row and stops at it, or as soon as it sees
+    // the flag set (by itself or by another lane; the decision is read after
+    // a barrier)
+    auto mark_fail = [&]() {
+        if (!act) return;
+        const volatile uint32_t* flag = cnt;   // re-read every layer
+        for (int l = 0; l < mb && *flag == 0; l++) {
+            uint32_t p = 0;
+            const int dg = d.deg[l];
+            for (int j = 0; j < dg; j++) p ^= rx[var(l, j)] >= 0.0f ? 1u : 0u;
+            if (p) {
+                cnt[0] = 1;
+                break;
+            }
+        }
+    };
+
     // one layer for this lane's row (lib/ldpc_decoder.cc:143-240)
     auto layer = [&](int l) {
         const int dg = d.deg[l];
@@ -89,7 +108,7 @@ __global__ __launch_bounds__(kGenMaxZ) void generic_kernel(GenArgs a)
     int it_done = 0;
     if (a.early_stop) {
         // initial hard decision + syndrome (lib/ldpc_decoder.cc:130-139)
-        if (fail_rows()) cnt[0] = 1;
+        mark_fail();
         __syncthreads();
         done = cnt[0] == 0;
         __syncthreads();
@@ -103,7 +122,7 @@ __global__ __launch_bounds__(kGenMaxZ) void generic_kernel(GenArgs a)
         }
         if (a.early_stop) {
             // lib/ldpc_decoder.cc:242-251
-            if (fail_rows()) cnt[0] = 1;
+            mark_fail();
             __syncthreads();
             if (cnt[0] == 0) {
                 done = true;
@@ -117,9 +136,15 @@ __global__ __launch_bounds__(kGenMaxZ) void generic_kernel(GenArgs a)
     const bool converged = a.early_stop && done;
     if (!converged) it_done = a.max_iter;
 
-    if (a.synd || a.ret) {
-        const int fr = fail_rows();
+    // Final syndrome: the weight where it is asked for (a frame that stopped
+    // early satisfies every check: weight 0); with fixed iterations the
+    // return value only needs to know whether a row fails; with early stop
+    // it follows from the stop decisions.
+    if (a.synd) {
+        const int fr = converged ? 0 : fail_rows();
         if (fr) atomicAdd(&cnt[0], (uint32_t)fr);
+    } else if (a.ret && !a.early_stop) {
+        mark_fail();
     }
     __syncthreads();
     if (tid == 0) {

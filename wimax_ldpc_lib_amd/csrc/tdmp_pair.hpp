@@ -261,9 +261,15 @@ __global__ __launch_bounds__(kPairMaxThreads) void tdmp_pair_kernel(DecodeArgs a
     }
     const bool converged = a.early_stop && done;
     if (!converged) it_done = a.max_iter;
-    if (a.synd || a.ret) {
-        const int fr = active ? fail_rows() : 0;
+    // Final syndrome: the weight where it is asked for (a frame that stopped
+    // early satisfies every check: weight 0); with fixed iterations the
+    // return value only needs to know whether a row fails; with early stop
+    // it follows from the stop decisions.
+    if (a.synd) {
+        const int fr = active && !converged ? fail_rows() : 0;
         if (fr) atomicAdd(rows_failing, (uint32_t)fr);
+    } else if (a.ret && !a.early_stop) {
+        if (frame_fails() && tid == 0) *rows_failing = 1;
     }
     __syncthreads();
     if (tid == 0) {
