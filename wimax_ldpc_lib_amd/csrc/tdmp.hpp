@@ -675,17 +675,30 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
         }
     }
     if (a.hard) {
-        // per frame: ballots of 64 consecutive variables, bytes written by lanes 0..7
+        // per frame: ballots of 64 consecutive variables, each the 8 bytes of
+        // packed hard decisions of its chunk, stored by lane 0 as two dwords
+        // (n / 8 = 3 z bytes per frame: rows are 4-byte aligned, and the last
+        // chunk of a frame holds 64 or 32 variables).  The chunk loop is
+        // uniform over the wave, so its indices and the frame's output
+        // address are scalar values.
         constexpr int kChunks = (N + 63) / 64;
+        static_assert(N % 32 == 0 && (N / 8) % 4 == 0, "packed rows of whole dwords");
         const int lane = tid & 63;
         const int nw = blockDim.x >> 6;
-        for (int q = tid >> 6; q < Feff * kChunks; q += nw) {
-            const int s = q / kChunks, c0 = (q - s * kChunks) * 64;
-            const int v = c0 + lane;
+        const int w0 = __builtin_amdgcn_readfirstlane(tid >> 6);
+        for (int q = w0; q < Feff * kChunks; q += nw) {
+            const int s = q / kChunks, c = q - s * kChunks;
+            const int v = 64 * c + lane;
             const bool bit = v < N && rxall[s * NS + v] >= 0.0f;
-            const unsigned long long m = __ballot(bit);
-            const int byte = (c0 >> 3) + lane;
-            if (lane < 8 && byte < N / 8) a.hard[fo(s) / 8 + byte] = (uint8_t)(m >> (8 * lane));
+            const uint64_t m = __builtin_amdgcn_ballot_w64(bit);
+            unsigned long long off = fo(s);
+            if constexpr (EXT) off = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)off)) |
+                                     ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(off >> 32)) << 32);
+            uint32_t* dst = reinterpret_cast<uint32_t*>(a.hard + off / 8 + 8 * c);
+            if (lane == 0) {
+                dst[0] = (uint32_t)m;
+                if (64 * c + 64 <= N) dst[1] = (uint32_t)(m >> 32);
+            }
         }
     }
 }
