@@ -58,8 +58,14 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // LDS byte address of a shared-memory pointer, and back: the address arithmetic
 // of the rotated row reads then stays in 32-bit LDS space, where a constant
-// term folds into the ds instruction's offset field.
-typedef __attribute__((address_space(3))) float lds_float;
+// term folds into the ds instruction's offset field.  The accesses are
+// volatile, which keeps them in program order (as the layers issue them
+// anyway) and stops the compiler from pairing two slots into one
+// ds_read2 / ds_write2: the pair's 8-bit offset fields reach 1020 bytes, so
+// pairing slots further apart cost a vector add for a new base address, and
+// the kernels are bound by vector-ALU issue, not LDS issue (rate 5/6 z=96:
+// 76 adds fewer per iteration, 1.8 % faster).
+typedef volatile __attribute__((address_space(3))) float lds_float;
 __device__ __forceinline__ uint32_t lds_addr(unsigned char* p)
 {
     return (uint32_t)(uintptr_t)(__attribute__((address_space(3))) unsigned char*)p;
@@ -513,7 +519,12 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             }
         } else {
             const int tt = t < a.n_offset_iters ? t : a.n_offset_iters - 1;
-            const float* beta = a.offsets + ((size_t)tt * NL + L) * C;
+            // the layer's offsets are uniform and read-only during the
+            // launch: scalar loads through the constant address space (as
+            // global loads they were re-issued around the volatile LDS
+            // accesses: 1.20 against 0.77 ms per 65536-frame decode)
+            const __attribute__((address_space(4))) float* beta =
+                (const __attribute__((address_space(4))) float*)(a.offsets + ((size_t)tt * NL + L) * C);
             if constexpr ((VAR & VAR_REREAD) != 0) asm volatile("" ::: "memory");
             static_for<0, D>([&](auto Jc) {
                 constexpr int J = decltype(Jc)::value;
@@ -605,9 +616,12 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
             if (tid < F) cnt[tid] = 0;
         }
         int any = __syncthreads_or(!done);
+        // lanes still decoding, as a wave-wide mask in scalar registers: the
+        // layer guard is then an exec-mask operation with no vector work
+        uint64_t live = __builtin_amdgcn_ballot_w64(!done);
         for (int it = 0; it < a.max_iter && any; it++) {
             static_for<0, NL>([&](auto Lc) {
-                if (!done) layer(Lc, it, Wc);
+                if (__builtin_amdgcn_inverse_ballot_w64(live)) layer(Lc, it, Wc);
                 __syncthreads();
             });
             if (a.early_stop) {
@@ -618,6 +632,7 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
                     done = true;
                     it_done = it + 1;
                 }
+                live = __builtin_amdgcn_ballot_w64(!done);
                 any = __syncthreads_or(!done);
                 if (tid < F) cnt[tid] = 0;
             }
