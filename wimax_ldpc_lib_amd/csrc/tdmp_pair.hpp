@@ -217,9 +217,34 @@ __global__ __launch_bounds__(kPairMaxThreads) void tdmp_pair_kernel(DecodeArgs a
         });
         return fails;
     };
-    // does any row of the frame fail (uniform over the workgroup)
+    // Whether some row of this wave's rows of layer L fails: row r's parity
+    // is bit 2r ^ bit 2r + 1 of the XORed ballots, masked to the active rows
+    // (a scalar test, no per-lane count).
+    const uint64_t act_rows = __builtin_amdgcn_ballot_w64(active) & 0x5555555555555555ull;
+    auto layer_fails = [&](auto Lc) -> bool {
+        constexpr int L = decltype(Lc)::value;
+        constexpr int D = CD::deg(L);
+        constexpr int H = (D + 1) / 2;
+        constexpr bool PAD = (D & 1) != 0;
+        float hv[H];
+        static_for<0, H>([&](auto Kc) { hv[decltype(Kc)::value] = *lds_ptr(ad[L][decltype(Kc)::value]); });
+        uint64_t pm = 0;
+        static_for<0, H>([&](auto Kc) {
+            constexpr int k = decltype(Kc)::value;
+            bool b = hv[k] >= 0.0f;
+            if constexpr (PAD && k == H - 1) b = b && !odd;
+            pm ^= __builtin_amdgcn_ballot_w64(b);
+        });
+        return ((pm ^ (pm >> 1)) & act_rows) != 0;
+    };
+    // does any row of the frame fail (uniform over the workgroup).  Only
+    // this is needed for the stop decisions (compute_syndrome(hd, true)
+    // returns at the first failing row), so layer 0 is checked first: a
+    // frame with errors almost always fails there, and the other layers'
+    // reads (all in flight at once) are issued only when it does not.
     auto frame_fails = [&]() -> bool {
-        const bool w = __builtin_amdgcn_ballot_w64(active && fail_rows() != 0) != 0;
+        bool w = layer_fails(std::integral_constant<int, 0>{});
+        if (!w) w = __builtin_amdgcn_ballot_w64(active && fail_rows() != 0) != 0;
         if constexpr (NW == 1) {
             return w;
         } else {
