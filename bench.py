@@ -36,13 +36,24 @@ USABLE_CPUS = len(os.sched_getaffinity(0))
 
 
 def cpu_model():
+    """Model name of the first CPU in /proc/cpuinfo, with its family / model /
+    stepping numbers (which name the generation where the model name is
+    generic, as on virtual machines)."""
+    fields = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" not in line:
+                if fields:
+                    break   # end of the first CPU's block
+                continue
+            key, val = (x.strip() for x in line.split(":", 1))
+            fields.setdefault(key, val)
     except OSError:
         pass
-    return "unknown"
+    name = fields.get("model name", "unknown")
+    ids = [("family", fields.get("cpu family")), ("model", fields.get("model")), ("stepping", fields.get("stepping"))]
+    ids = ", ".join("%s %s" % (k, v) for k, v in ids if v)
+    return "%s (%s)" % (name, ids) if ids else name
 
 
 def cpu_env(threads):
