@@ -341,6 +341,49 @@ def test_nan_input_stays_in_its_frame(gpu, oracle_mod):
                 _compare(gk, o, "NaN batch rate %d z %d early %d small_batch %r" % (rate, z, early, small))
 
 
+@pytest.mark.parametrize("rate, z", [(0, 24), (0, 96), (2, 56), (5, 32)])
+def test_failures_only_in_late_layers(gpu, oracle_mod, rate, z):
+    """The stop checks exit once a frame is known to fail (after layers 0, 1,
+    3 and 7 in the row-per-lane kernel, after layer 0 in the small-batch
+    kernel), and the fixed-iteration return value uses the same early-exit
+    check; a frame whose only failing rows lie in the last base rows is the
+    case where no early layer decides.  Strong noiseless LLRs of random
+    codewords with one or two bits flipped in the last parity block column,
+    which only the last two base rows touch: every kernel and stop mode,
+    with and without the syndrome weight output, equals the oracle."""
+    import wimax_ldpc_lib_amd as W
+    torch = gpu
+    code = oracle_mod.Code(rate, z)
+    col0 = (24 - 1) * z   # first variable of the last block column
+    rows = [m for m in range(code.m) if any(code.cn[m, j] == col0 for j in range(code.row_len[m]))]
+    assert rows and min(rows) // z >= code.mb - 2, rows   # touched by the last two base rows only
+    F = 67
+    rng = np.random.default_rng(rate * 1000 + z)
+    cw = code.encode(rng.integers(0, 2, (F, code.k), dtype=np.uint8))
+    llr = np.where(cw == 1, 8.0, -8.0).astype(np.float32)
+    for f in range(F):
+        for t in {(3 * f) % z, (7 * f + 1) % z} if f % 3 == 0 else {(5 * f) % z}:
+            llr[f, col0 + t] = -llr[f, col0 + t]
+    for early, max_iter in ((True, 10), (True, 0), (False, 0), (False, 2)):
+        o = code.decode(llr, max_iter, early)
+        assert early or max_iter > 0 or (o["ret"] == 1).all()   # every frame fails its checks before decoding
+        dec = W.ldpc_decoder(rate, z, max_iter, early_stop=early)
+        for small in (None, 0):
+            dec.small_batch = small
+            label = "late-layer failures rate %d z %d early %d max_iter %d small_batch %r" % (
+                rate, z, early, max_iter, small)
+            _compare(_gpu_decode(torch, dec, llr), o, label)
+            # return value without the syndrome weight output
+            d_llr = torch.from_numpy(llr).cuda()
+            hard = torch.empty((F, code.n // 8), dtype=torch.uint8, device="cuda")
+            ret = torch.empty(F, dtype=torch.uint8, device="cuda")
+            iters = torch.empty(F, dtype=torch.uint8, device="cuda")
+            dec.decode_dev(d_llr, None, hard, ret, iters)
+            torch.cuda.synchronize()
+            assert np.array_equal(ret.cpu().numpy(), o["ret"]), label + " ret without synd"
+            assert np.array_equal(iters.cpu().numpy(), o["iters"]), label + " iters without synd"
+
+
 @pytest.mark.parametrize("rule", ["oms", "ms"])
 @pytest.mark.parametrize("rate", range(6))
 def test_offset_and_plain_min_sum_rules(gpu, oracle_mod, rule, rate):
