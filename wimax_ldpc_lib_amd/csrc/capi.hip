@@ -134,6 +134,47 @@ struct DevBuf {
     }
 };
 
+// Pinned host memory (hipHostMalloc: mapped into the device's address
+// space) that kernels read and write directly.  The host calls of small
+// batches stage through it: a call is then a host copy in, one kernel and a
+// synchronisation, with no DMA transfer of pageable memory.
+template <class T>
+struct PinnedBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t n)
+    {
+        if (n <= cap) return WL_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipHostMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault);
+        if (e != hipSuccess) return hip_fail(e, "hipHostMalloc");
+        cap = n;
+        return WL_OK;
+    }
+    ~PinnedBuf()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+// Host calls up to this many bytes of input go through pinned memory the
+// kernels access over PCIe; larger ones are copied to device buffers first.
+constexpr size_t kPinnedCallBytes = 4u << 20;
+
+// the handle's stream for host calls, created on first use
+int host_stream(hipStream_t* s)
+{
+    if (*s) return WL_OK;
+    hipError_t e = hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        *s = nullptr;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    return WL_OK;
+}
+
 // fallback list header {first slot 0, count 0}
 __global__ void zero_bucket_kernel(uint32_t* p)
 {
@@ -184,7 +225,14 @@ struct wl_decoder {
     DevBuf<wl::GenDesc> d_gen;
     // fallback scratch
     DevBuf<uint32_t> d_idx, d_count;
-    hipStream_t stream = nullptr;
+    // small host calls: posteriors in place in pinned memory, then status
+    PinnedBuf<float> h_llr;
+    PinnedBuf<uint8_t> h_stat;
+    hipStream_t stream = nullptr;   // host calls
+    ~wl_decoder()
+    {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
 };
 
 // packed hard decisions [nframes][n/8] (bit v%8 of byte v/8) -> one 0/1 byte
@@ -218,6 +266,12 @@ struct wl_encoder {
     HostCode code;
     wl::EncodeDesc desc{};
     DevBuf<uint8_t> d_info, d_cw;
+    PinnedBuf<uint8_t> h_info, h_cw;   // small host calls
+    hipStream_t stream = nullptr;      // host calls
+    ~wl_encoder()
+    {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
 };
 
 extern "C" {
@@ -623,6 +677,26 @@ int wl_decode_f32_host(wl_decoder* dec, float* rx, size_t nframes, int8_t* decod
     const int n = dec->code.info.n;
     const size_t nb = ((size_t)n + 7) / 8;
     int rc;
+    if (nframes * n * sizeof(float) <= kPinnedCallBytes) {
+        // small batch (decode() is one frame): the kernel reads the LLRs from
+        // pinned memory and writes the posteriors back in place and the
+        // status beside them; the 0/1 decisions are the posteriors' signs,
+        // sliced here as the GPU slices them (rx >= 0, :130,242)
+        const size_t len = nframes * (size_t)n;
+        if ((rc = dec->h_llr.ensure(len)) || (rc = dec->h_stat.ensure(2 * nframes)) || (rc = host_stream(&dec->stream)))
+            return rc;
+        memcpy(dec->h_llr.p, rx, len * sizeof(float));
+        uint8_t* hret = dec->h_stat.p;
+        uint8_t* hit = dec->h_stat.p + nframes;
+        rc = wl_decode_f32_dev(dec, dec->h_llr.p, nframes, dec->h_llr.p, nullptr, hret, hit, nullptr, dec->stream);
+        if (rc) return rc;
+        WL_HIP(hipStreamSynchronize(dec->stream));
+        memcpy(rx, dec->h_llr.p, len * sizeof(float));
+        for (size_t i = 0; i < len; i++) decoded[i] = rx[i] >= 0.0f ? 1 : 0;
+        memcpy(ret, hret, nframes);
+        if (iters) memcpy(iters, hit, nframes);
+        return WL_OK;
+    }
     if ((rc = dec->d_llr.ensure(nframes * n)) || (rc = dec->d_hard.ensure(nframes * nb)) ||
         (rc = dec->d_ret.ensure(nframes)) || (rc = dec->d_iters.ensure(nframes)) ||
         (rc = dec->d_dec.ensure(nframes * n)))
@@ -804,6 +878,19 @@ int wl_encode_u8_host(wl_encoder* enc, const uint8_t* info, size_t nframes, uint
     if (!info || !codeword) return fail(WL_EINVAL, "null buffer");
     const size_t k = enc->code.info.k, n = enc->code.info.n;
     int rc;
+    if (nframes * n <= kPinnedCallBytes) {
+        // small batch (encode_data() is one frame): the kernel reads and
+        // writes pinned memory
+        if ((rc = enc->h_info.ensure(nframes * k)) || (rc = enc->h_cw.ensure(nframes * n)) ||
+            (rc = host_stream(&enc->stream)))
+            return rc;
+        memcpy(enc->h_info.p, info, nframes * k);
+        rc = wl_encode_u8_dev(enc, enc->h_info.p, nframes, enc->h_cw.p, enc->stream);
+        if (rc) return rc;
+        WL_HIP(hipStreamSynchronize(enc->stream));
+        memcpy(codeword, enc->h_cw.p, nframes * n);
+        return WL_OK;
+    }
     if ((rc = enc->d_info.ensure(nframes * k)) || (rc = enc->d_cw.ensure(nframes * n))) return rc;
     WL_HIP(hipMemcpy(enc->d_info.p, info, nframes * k, hipMemcpyHostToDevice));
     rc = wl_encode_u8_dev(enc, enc->d_info.p, nframes, enc->d_cw.p, nullptr);
