@@ -454,8 +454,12 @@ hipError_t launch_encode(const EncodeDesc& d, const uint8_t* info, uint8_t* cw,
 {
     if (nframes == 0) return hipSuccess;
     const uintptr_t al = (uintptr_t)info | (uintptr_t)cw;
-    // compiled codes: the bit-sliced kernel, 64 frames per workgroup
-    if (d.compiled && d.rate >= 0 && d.rate < kNumRates && valid_z(d.z)) {
+    // compiled codes: the bit-sliced kernel, 64 frames per workgroup, from
+    // one full tile on (a smaller batch leaves most bits of every word
+    // unused, and its chunk steps, one load round trip and two barriers each,
+    // make it slower for a single frame than the byte-per-frame kernel, whose
+    // loads are all in flight at once)
+    if (d.compiled && d.rate >= 0 && d.rate < kNumRates && valid_z(d.z) && nframes >= (unsigned long long)kBsFrames) {
         const EncodeBsEntry& e = kEncodeBsArr.e[d.rate * kNumZ + index_of_z(d.z)];
         if (al % (uintptr_t)e.vec == 0) {
             const unsigned long long grid = (nframes + kBsFrames - 1) / kBsFrames;
