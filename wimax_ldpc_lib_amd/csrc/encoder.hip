@@ -131,7 +131,6 @@ constexpr int kBsThreads = 256;
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
-
 // bytes per frame staged per step: the largest multiple of VEC that divides
 // k and is at most 128 (four waves of at most 32 positions each)
 template <int K, int VEC>
