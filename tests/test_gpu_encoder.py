@@ -18,23 +18,27 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 @pytest.mark.parametrize("rate", [0, 1, 2, 3, 5])
 def test_encoder_matches_oracle(gpu, oracle_mod, rate):
+    """Every z of the rate, through both encoder kernels: 97 frames (one full
+    64-frame bit-sliced tile and a ragged one) and 33 frames (below one
+    tile: the byte-per-frame kernel), device and host calls."""
     import wimax_ldpc_lib_amd as W
     torch = gpu
     for z in Z_VALUES:
         code = oracle_mod.Code(rate, z)
         rng = np.random.default_rng(rate * 100 + z)
-        info = rng.integers(0, 2, (33, code.k), dtype=np.uint8)
-        info[0, :] = rng.integers(0, 256, code.k, dtype=np.uint8)   # nonzero bytes mean 1, copied as is
-        want = code.encode(info)
         enc = W.ldpc_encoder(rate, z)
-        d_info = torch.from_numpy(info).cuda()
-        d_cw = torch.empty((33, code.n), dtype=torch.uint8, device="cuda")
-        enc.encode_dev(d_info, d_cw)
-        torch.cuda.synchronize()
-        assert np.array_equal(d_cw.cpu().numpy(), want), (rate, z)
-        host = np.zeros((33, code.n), np.uint8)
-        enc.encode_data(info, host)
-        assert np.array_equal(host, want)
+        for F in (97, 33):
+            info = rng.integers(0, 2, (F, code.k), dtype=np.uint8)
+            info[0, :] = rng.integers(0, 256, code.k, dtype=np.uint8)   # nonzero bytes mean 1, copied as is
+            want = code.encode(info)
+            d_info = torch.from_numpy(info).cuda()
+            d_cw = torch.empty((F, code.n), dtype=torch.uint8, device="cuda")
+            enc.encode_dev(d_info, d_cw)
+            torch.cuda.synchronize()
+            assert np.array_equal(d_cw.cpu().numpy(), want), (rate, z, F)
+            host = np.zeros((F, code.n), np.uint8)
+            enc.encode_data(info, host)
+            assert np.array_equal(host, want), (rate, z, F)
 
 
 @pytest.mark.parametrize("rate, z", [(0, 96), (3, 28), (5, 24), (4, 52), (1, 64), (2, 36)])
@@ -86,11 +90,13 @@ def test_encoder_3_4b_matches_reference_behaviour(gpu, oracle_mod):
     """3/4B: the reference encoder runs but its codewords fail H (as the
     reference's own encoder does); the GPU reproduces those bytes exactly."""
     import wimax_ldpc_lib_amd as W
-    code = oracle_mod.Code(4, 96)
-    info = np.random.default_rng(0).integers(0, 2, (4, code.k), dtype=np.uint8)
-    got = np.zeros((4, code.n), np.uint8)
-    W.ldpc_encoder(4, 96).encode_data(info, got)
-    assert np.array_equal(got, code.encode(info))
+    for z in (96, 28):
+        code = oracle_mod.Code(4, z)
+        for F in (4, 70):   # the byte-per-frame and the bit-sliced kernel
+            info = np.random.default_rng(z + F).integers(0, 2, (F, code.k), dtype=np.uint8)
+            got = np.zeros((F, code.n), np.uint8)
+            W.ldpc_encoder(4, z).encode_data(info, got)
+            assert np.array_equal(got, code.encode(info)), (z, F)
 
 
 def _philox(c, k0, k1):
@@ -183,22 +189,22 @@ def test_standard_encoder(gpu, oracle_mod, rate):
     for z in Z_VALUES:
         code = oracle_mod.Code(rate, z)
         rng = np.random.default_rng(900 + rate * 100 + z)
-        info = rng.integers(0, 2, (17, code.k), dtype=np.uint8)
         enc = W.ldpc_encoder(rate, z, standard=True)
-        d_info = torch.from_numpy(info).cuda()
-        d_cw = torch.empty((17, code.n), dtype=torch.uint8, device="cuda")
-        enc.encode_dev(d_info, d_cw)
-        torch.cuda.synchronize()
-        cw = d_cw.cpu().numpy()
-        assert np.array_equal(cw[:, :code.k], info)
-        for f in range(17):
-            assert code.syndrome(cw[f].astype(np.int8)) == 0, (rate, z, f)
-        if rate != 4:
-            assert np.array_equal(cw, code.encode(info)), (rate, z)
-        elif z in (24, 48):
-            H = code.dense_h()
-            for f in range(3):
-                assert np.array_equal(cw[f, code.k:], _gf2_parity(H, code.k, info[f])), (z, f)
+        for F in (17, 81):   # the byte-per-frame and the bit-sliced kernel
+            info = rng.integers(0, 2, (F, code.k), dtype=np.uint8)
+            d_info = torch.from_numpy(info).cuda()
+            d_cw = torch.empty((F, code.n), dtype=torch.uint8, device="cuda")
+            enc.encode_dev(d_info, d_cw)
+            torch.cuda.synchronize()
+            cw = d_cw.cpu().numpy()
+            assert np.array_equal(cw[:, :code.k], info)
+            assert (code.dense_h().astype(np.int64) @ cw.T.astype(np.int64) % 2 == 0).all(), (rate, z, F)
+            if rate != 4:
+                assert np.array_equal(cw, code.encode(info)), (rate, z, F)
+            elif z in (24, 48):
+                H = code.dense_h()
+                for f in (0, 1, F - 1):
+                    assert np.array_equal(cw[f, code.k:], _gf2_parity(H, code.k, info[f])), (z, F, f)
     # BPSK / AWGN frames of the standard encoder at 6 dB decode to what was sent
     z = 48
     enc = W.ldpc_encoder(rate, z, standard=True)
