@@ -422,6 +422,19 @@ def test_offset_and_plain_min_sum_rules(gpu, oracle_mod, rule, rate):
             _compare(g, {k: v[:nf] for k, v in o.items() if k != "seconds"}, "%s F=%d nf=%d" % (rule, f, nf))
         f += 1
     assert f > 2
+    # two frames per workgroup at z % 32 == 0 (the two-halves code shape),
+    # an odd frame count (a half-filled last workgroup)
+    for z in (32, 64, 96):
+        code = oracle_mod.Code(rate, z)
+        if rate == 4:
+            llr = np.random.default_rng(400 + z).normal(1.0, 2.0, (37, code.n)).astype(np.float32)
+        else:
+            llr, _ = noisy_llr(code, 37, ebn0, seed=91 + 1000 * rate + z)
+        for early in (True, False):
+            o = code.decode(llr, max_iter=6, early_stop=early, variant=variant)
+            dec = W.ldpc_decoder(rate, z, 6, early_stop=early, rule=rule)
+            dec.frames_per_wg = 2
+            _compare(_gpu_decode(gpu, dec, llr), o, "%s F=2 z %d early %d" % (rule, z, early))
     # special inputs: clean codewords, |L| >= 10000, signed zeros, max_iter 0
     if rate != 4:
         code = oracle_mod.Code(rate, 96)

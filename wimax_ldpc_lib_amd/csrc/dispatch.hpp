@@ -34,13 +34,33 @@ struct DecodeEntry {
     DecodeLaunch learned_fb;         // learned + in-kernel normalised min-sum re-queue
                                      // (persistent only; built for z = 24 and 96, else null)
     DecodeOccupancy occ_learned_fb;
-    DecodeLaunch alt;                // offset / plain min-sum (DecodeArgs::rule), register-lean shape
+    DecodeLaunch alt;                // offset / plain min-sum (DecodeArgs::rule)
 };
+
+// learned + re-queue: the two-halves shape where the decoder runs two frames
+// per workgroup and z % 32 == 0 (the shape tuned for every z = 96 code; both
+// the learned and the re-queued tiles use it), the register-lean shape
+// otherwise
+template <int RATE, int Z>
+hipError_t launch_learned_fb(const DecodeArgs& a, hipStream_t stream)
+{
+    if constexpr (Z % 32 == 0)
+        if (a.frames_per_wg == 2) return launch_tdmp<RATE, Z, MODE_LEARNED_FB, VAR_LEAN_W2>(a, stream);
+    return launch_tdmp<RATE, Z, MODE_LEARNED_FB, VAR_LEAN>(a, stream);
+}
+
+template <int RATE, int Z>
+int occupancy_learned_fb(int frames_per_wg)
+{
+    if constexpr (Z % 32 == 0)
+        if (frames_per_wg == 2) return occupancy_tdmp<RATE, Z, MODE_LEARNED_FB, VAR_LEAN_W2, true>(frames_per_wg);
+    return occupancy_tdmp<RATE, Z, MODE_LEARNED_FB, VAR_LEAN, true>(frames_per_wg);
+}
 
 template <int RATE, int Z>
 constexpr DecodeLaunch fb_launcher()
 {
-    if constexpr (Z == 24 || Z == 96) return &launch_tdmp<RATE, Z, MODE_LEARNED_FB, VAR_LEAN>;
+    if constexpr (Z == 24 || Z == 96) return &launch_learned_fb<RATE, Z>;
     else return nullptr;
 }
 
@@ -83,10 +103,21 @@ hipError_t launch_learned(const DecodeArgs& a, hipStream_t stream)
     return launch_tdmp<RATE, Z, MODE_LEARNED, 0>(a, stream);
 }
 
+// offset / plain min-sum: the two-halves shape where the decoder runs two
+// frames per workgroup and z % 32 == 0 (as the other rules there), the
+// register-lean shape otherwise
+template <int RATE, int Z>
+hipError_t launch_alt(const DecodeArgs& a, hipStream_t stream)
+{
+    if constexpr (Z % 32 == 0)
+        if (a.frames_per_wg == 2) return launch_tdmp<RATE, Z, MODE_ALT, VAR_LEAN_W2>(a, stream);
+    return launch_tdmp<RATE, Z, MODE_ALT, VAR_LEAN>(a, stream);
+}
+
 template <int RATE, int Z>
 constexpr DecodeOccupancy fb_occupancy()
 {
-    if constexpr (Z == 24 || Z == 96) return &occupancy_tdmp<RATE, Z, MODE_LEARNED_FB, VAR_LEAN, true>;
+    if constexpr (Z == 24 || Z == 96) return &occupancy_learned_fb<RATE, Z>;
     else return nullptr;
 }
 
@@ -128,7 +159,7 @@ constexpr auto make_decode_table(std::integer_sequence<int, ZI...>)
                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_LEARNED, 0, true>,
                             fb_launcher<RATE, z_of_index(ZI)>(),
                             fb_occupancy<RATE, z_of_index(ZI)>(),
-                            &launch_tdmp<RATE, z_of_index(ZI), MODE_ALT, VAR_LEAN>}...}};
+                            &launch_alt<RATE, z_of_index(ZI)>}...}};
 }
 
 }  // namespace wl

@@ -13,11 +13,18 @@ steps, `--eval-frames` per point, decoded by
   learned-5  the trained offsets, 5 iterations
   nms-5      the reference's normalised min-sum (0.815), 5 iterations
   nms-10     the same, 10 iterations (the reference's MAX_ITER)
+  ms-10      plain min-sum, 10 iterations (the reference's compile-time
+             alternative, lib/ldpc_decoder.cc:21,233-238)
+  oms-10     offset min-sum with the reference's offset 0.10, 10 iterations
+             (lib/ldpc_decoder.cc:20,225-232)
+  learned-5+fb  learned-5, frames failing its syndrome check decoded again
+             from their channel LLRs by nms-10 (the fallback)
 all with the reference's syndrome early stop.  BER counts info-bit errors
 over the k systematic bits; FER counts frames with any info-bit error.  The
 Eb/N0 at BER 1e-5 is interpolated log-linearly between the straddling points.
-Throughput (info Gbit/s) of learned-5 and nms-10 is timed on 65536 frames at
-`--speed-ebn0`, both with early stop and with fixed iterations.
+Throughput (info Gbit/s) of learned-5, learned-5+fb, nms-10 and ms-10 is timed on
+65536 frames at `--speed-ebn0` and at nms-10's BER-1e-5 point, with early
+stop and with fixed iterations.
 
     python -m wimax_ldpc_lib_amd.learned_eval --out results.json [--steps 20000]
 """
@@ -62,26 +69,35 @@ def evaluate(torch, W, rate, z, offsets, ebn0s, frames, chunk, seed, T, log):
     decs["learned-%d" % T] = d
     decs["nms-%d" % T] = W.ldpc_decoder(rate, z, T, early_stop=True)
     decs["nms-10"] = W.ldpc_decoder(rate, z, 10, early_stop=True)
+    decs["ms-10"] = W.ldpc_decoder(rate, z, 10, early_stop=True, rule="ms")
+    decs["oms-10"] = W.ldpc_decoder(rate, z, 10, early_stop=True, rule="oms")
+    decs["learned-%d+fb" % T] = d
     llr = torch.empty((chunk, n), dtype=torch.float32, device="cuda")
     bits = torch.empty((chunk, n), dtype=torch.uint8, device="cuda")
     hard = torch.empty((chunk, n // 8), dtype=torch.uint8, device="cuda")
     ret = torch.empty(chunk, dtype=torch.uint8, device="cuda")
     iters = torch.empty(chunk, dtype=torch.uint8, device="cuda")
+    fb = torch.empty(chunk, dtype=torch.uint8, device="cuda")
     weights = (1 << torch.arange(8, dtype=torch.int32, device="cuda")).view(1, 1, 8)   # LSB-first packing
     points = []
     for pi, eb in enumerate(ebn0s):
-        acc = {name: dict(bit_err=0, frame_err=0, coded_err=0, iters=0, fail=0) for name in decs}
+        acc = {name: dict(bit_err=0, frame_err=0, coded_err=0, iters=0, fail=0, fallback=0) for name in decs}
         done = 0
         while done < frames:
             m = min(chunk, frames - done)
             enc.channel_dev(seed, pi * (1 << 32) + done, m, eb, llr=llr[:m], bits=bits[:m])
             ref = (bits[:m].view(m, n // 8, 8).int() * weights).sum(dim=2).to(torch.uint8)
             for name, dec in decs.items():
-                dec.decode_dev(llr[:m], None, hard[:m], ret[:m], iters[:m])
+                a = acc[name]
+                if name.endswith("+fb"):
+                    dec.decode_learned_fallback_dev(llr[:m], 10, None, hard[:m], ret[:m], iters[:m],
+                                                    fallback=fb[:m])
+                    a["fallback"] += int(fb[:m].int().sum())
+                else:
+                    dec.decode_dev(llr[:m], None, hard[:m], ret[:m], iters[:m])
                 x = hard[:m] ^ ref
                 ie = popcount_rows(torch, x[:, :k // 8])
                 ce = popcount_rows(torch, x)
-                a = acc[name]
                 a["bit_err"] += int(ie.sum())
                 a["frame_err"] += int((ie > 0).sum())
                 a["coded_err"] += int(ce.sum())
@@ -93,36 +109,54 @@ def evaluate(torch, W, rate, z, offsets, ebn0s, frames, chunk, seed, T, log):
             p[name] = {"ber": a["bit_err"] / (frames * k), "fer": a["frame_err"] / frames,
                        "ber_runner_def": a["coded_err"] / (frames * k),
                        "avg_iters": a["iters"] / frames, "syndrome_fail_rate": a["fail"] / frames}
+            if name.endswith("+fb"):
+                p[name]["fallback_fraction"] = a["fallback"] / frames
         points.append(p)
         log("Eb/N0 %.2f dB: " % eb + "  ".join("%s BER %.3g FER %.3g it %.2f" % (
             nm, p[nm]["ber"], p[nm]["fer"], p[nm]["avg_iters"]) for nm in decs))
     return points
 
 
-def speed(torch, W, rate, z, offsets, T, eb, frames, reps=5):
+def speed(torch, W, rate, z, offsets, T, eb, frames, reps=10):
     enc = W.ldpc_encoder(rate, z)
     n, k = enc.n, enc.k
     llr = torch.empty((frames, n), dtype=torch.float32, device="cuda")
     enc.channel_dev(77, 0, frames, eb, llr=llr)
     hard = torch.empty((frames, n // 8), dtype=torch.uint8, device="cuda")
     ret = torch.empty(frames, dtype=torch.uint8, device="cuda")
-    out = {}
+    fb = torch.empty(frames, dtype=torch.uint8, device="cuda")
+    # about a second of decoding first, so every timed leg runs at the
+    # clocks a sustained load settles at (the first legs otherwise read slow)
+    warm = W.ldpc_decoder(rate, z, 10, early_stop=False)
+    for _ in range(400):
+        warm.decode_dev(llr, None, hard, ret)
+    torch.cuda.synchronize()
+    out = {"ebn0": eb}
     for early in (False, True):
-        for name, it, off in (("learned-%d" % T, T, offsets), ("nms-10", 10, None)):
-            d = W.ldpc_decoder(rate, z, it, early_stop=early)
+        for name, it, off in (("learned-%d" % T, T, offsets), ("learned-%d+fb" % T, T, offsets),
+                              ("nms-10", 10, None), ("ms-10", 10, None)):
+            d = W.ldpc_decoder(rate, z, it, early_stop=early, rule="ms" if name == "ms-10" else "nms")
             if off is not None:
                 d.set_offsets(off)
-            d.decode_dev(llr, None, hard, ret)
+            if name.endswith("+fb"):
+                def run():
+                    d.decode_learned_fallback_dev(llr, 10, None, hard, ret, fallback=fb)
+            else:
+                def run():
+                    d.decode_dev(llr, None, hard, ret)
+            run()
             torch.cuda.synchronize()
             s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             s.record()
             for _ in range(reps):
-                d.decode_dev(llr, None, hard, ret)
+                run()
             e.record()
             torch.cuda.synchronize()
             ms = s.elapsed_time(e) / reps
-            out["%s%s" % (name, " early-stop" if early else " fixed")] = {
-                "ms": ms, "info_gbps": frames * k / (ms * 1e-3) / 1e9}
+            r = {"ms": ms, "info_gbps": frames * k / (ms * 1e-3) / 1e9}
+            if name.endswith("+fb"):
+                r["fallback_fraction"] = float(fb.float().mean())
+            out["%s%s" % (name, " early-stop" if early else " fixed")] = r
     return out
 
 
@@ -191,16 +225,39 @@ def main():
     t0 = time.time()
     pts = evaluate(torch, W, a.rate, a.z, offsets, ebs, a.eval_frames, a.chunk, a.seed + 1, a.T, log)
     result["eval"] = {"frames_per_point": a.eval_frames, "points": pts, "seconds": time.time() - t0}
-    th = {name: threshold_db(pts, name) for name in ("learned-%d" % a.T, "nms-%d" % a.T, "nms-10")}
+    th = {name: threshold_db(pts, name) for name in pts[0] if name not in ("ebn0", "frames")}
     result["ebn0_at_ber_1e-5"] = th
     sp = speed(torch, W, a.rate, a.z, offsets, a.T, a.speed_ebn0, a.chunk)
     result["speed"] = sp
-    L, N = th["learned-%d" % a.T], th["nms-10"]
+    L, LF, N = th["learned-%d" % a.T], th["learned-%d+fb" % a.T], th["nms-10"]
+
+    def gap(x, y):
+        return (x - y) if (x is not None and y is not None) else None
+
+    def ratio(sp, name, mode, base="nms-10"):
+        return sp["%s %s" % (name, mode)]["info_gbps"] / sp["%s %s" % (base, mode)]["info_gbps"]
+
+    lt, lf = "learned-%d" % a.T, "learned-%d+fb" % a.T
     result["summary"] = {
-        "gap_db_learned_vs_nms10": (L - N) if (L is not None and N is not None) else None,
-        "speedup_fixed": sp["learned-%d fixed" % a.T]["info_gbps"] / sp["nms-10 fixed"]["info_gbps"],
-        "speedup_early_stop": sp["learned-%d early-stop" % a.T]["info_gbps"] / sp["nms-10 early-stop"]["info_gbps"],
+        "gap_db_learned_vs_nms10": gap(L, N),
+        "gap_db_learned_vs_ms10": gap(L, th["ms-10"]),
+        "gap_db_learned_vs_oms10": gap(L, th["oms-10"]),
+        "gap_db_learned_fb_vs_nms10": gap(LF, N),
+        "speedup_fixed": ratio(sp, lt, "fixed"),
+        "speedup_early_stop": ratio(sp, lt, "early-stop"),
+        "speedup_fixed_vs_ms10": ratio(sp, lt, "fixed", "ms-10"),
+        "speedup_early_stop_vs_ms10": ratio(sp, lt, "early-stop", "ms-10"),
     }
+    if N is not None:
+        op = speed(torch, W, a.rate, a.z, offsets, a.T, round(N, 3), a.chunk)
+        result["speed_at_nms10_ber_1e-5"] = op
+        result["summary"].update({
+            "op_ebn0_db": op["ebn0"],
+            "op_speedup_learned_fixed": ratio(op, lt, "fixed"),
+            "op_speedup_learned_fb_fixed": ratio(op, lf, "fixed"),
+            "op_speedup_learned_fb_early_stop": ratio(op, lf, "early-stop"),
+            "op_fallback_fraction": op["%s early-stop" % lf]["fallback_fraction"],
+        })
     log(json.dumps(result["summary"]), json.dumps(th))
     if a.out:
         json.dump(result, open(a.out, "w"), indent=1)
