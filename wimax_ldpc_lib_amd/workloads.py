@@ -12,7 +12,11 @@
          min-sum (10 iterations) on the same frames: info Gbit/s, fallback
          fraction, BER / FER per rate
 
-    python -m wimax_ldpc_lib_amd.workloads mixed|fallback [--out file.json]
+    python -m wimax_ldpc_lib_amd.workloads mixed|fallback|groups [--out file.json]
+
+  groups per lifting size, the mixed graph over the five rates of that z
+         against the sum of the five per-code decodes at full occupancy, and
+         that per-code sum scaled to the mixed workload's draw
 
 (the small rate 1/2, z = 24 workload, measured against the CPU decoder, is
 bench.py --workload small)
@@ -109,6 +113,54 @@ def mixed(a, torch, W, np):
     return out
 
 
+def groups(a, torch, W, np):
+    """Per lifting size z: the mixed graph over the five rates of that z
+    alone (5 x F frames, rate uniform, Eb/N0 uniform in [1.0, 4.5] dB, 10
+    iterations with early stop) against the sum of the five per-code decodes
+    of F frames each, every code alone on its own reliability-sorted frames
+    (mean |LLR| of the first 256 LLRs) at its tuned shape: the cost of
+    grouping at a size where every launch fills the chip.  The sum of the
+    per-code times scaled to the mixed workload's draw (262144 frames over
+    95 codes) is the full-occupancy figure the mixed graph is compared with."""
+    from wimax_ldpc_lib_amd.mixed import MixedDecoder, code_id
+    F = a.frames or 65536
+    zs = list(range(24, 97, 4))
+    draw = np.random.default_rng(a.seed).choice(
+        np.array([code_id(r, z) for r in RATES5 for z in zs], dtype=np.uint8), 262144)
+    rows, bound = [], 0.0
+    for z in zs:
+        per = 0.0
+        for r in RATES5:
+            enc = W.ldpc_encoder(r, z)
+            llr = torch.empty((F, enc.n), dtype=torch.float32, device="cuda")
+            enc.channel_dev(a.seed + 100 * r + z, 0, F, 1.0, 4.5, llr=llr)
+            llr = llr.index_select(0, torch.argsort(llr[:, :256].abs().mean(dim=1))).contiguous()
+            hard = torch.empty((F, enc.n // 8), dtype=torch.uint8, device="cuda")
+            dec = W.ldpc_decoder(r, z, 10, early_stop=True)
+            dec.small_batch = 0
+            ms = timed(torch, lambda: dec.decode_dev(llr, None, hard), a.reps)
+            per += ms
+            bound += ms * int((draw == code_id(r, z)).sum()) / F
+            del llr, hard
+        codes = [code_id(r, z) for r in RATES5]
+        md = MixedDecoder(max_iter=10, early_stop=True, codes=codes)
+        cfg_np, cfg, off, llr, bits, n_of = mixed_inputs(torch, md, np, 5 * F, codes, a.seed + z, 1.0, 4.5)
+        hard = torch.empty(llr.numel() // 8, dtype=torch.uint8, device="cuda")
+        ret = torch.empty(5 * F, dtype=torch.uint8, device="cuda")
+        g = timed(torch, lambda: md.decode_dev(cfg, off, llr, None, hard, ret), a.reps)
+        md.close()
+        del llr, bits, hard
+        rows.append({"z": z, "group_graph_ms": g, "per_code_sum_ms": per, "ratio": g / per})
+        log("z %2d: group graph %.3f ms, per-code sum %.3f ms, ratio %.3f" % (z, g, per, g / per))
+    out = {"frames_per_code": F, "per_z": rows,
+           "group_total_ms": sum(r["group_graph_ms"] for r in rows),
+           "per_code_total_ms": sum(r["per_code_sum_ms"] for r in rows),
+           "per_code_full_occupancy_ms_for_the_mixed_draw": bound}
+    out["ratio"] = out["group_total_ms"] / out["per_code_total_ms"]
+    log(json.dumps({k: v for k, v in out.items() if k != "per_z"}))
+    return out
+
+
 def offsets_for(rate, a, np, log):
     path = os.path.join(PKG, "trained", "offsets_r%d_z96_T5.json" % rate)
     if os.path.exists(path):
@@ -179,7 +231,7 @@ def fallback(a, torch, W, np):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["mixed", "fallback"])
+    ap.add_argument("which", choices=["mixed", "fallback", "groups"])
     ap.add_argument("--frames", type=int, default=0)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--seed", type=int, default=31337)
@@ -191,7 +243,7 @@ def main():
     import torch
 
     import wimax_ldpc_lib_amd as W
-    out = {"mixed": mixed, "fallback": fallback}[a.which](a, torch, W, np)
+    out = {"mixed": mixed, "fallback": fallback, "groups": groups}[a.which](a, torch, W, np)
     if a.out:
         json.dump(out, open(a.out, "w"), indent=1)
 
