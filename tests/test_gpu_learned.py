@@ -338,3 +338,76 @@ def test_waterfall_trained_offsets_beat_nms5(gpu, oracle_mod):
         (lb, lf), (nb, nf) = errs["learned"], errs["nms5"]
         assert nf > 1000, errs            # enough frame errors to compare
         assert lf < 0.93 * nf and lb < nb, (eb, errs)
+
+
+def test_mixed_learned_fallback_full_size_524288_frames(gpu):
+    """The learned-with-fallback workload at its full size: 524288 frames,
+    the five encoder rates at z = 96, Eb/N0 uniform in [1.5, 5] dB, the
+    committed trained offsets of each rate, one captured graph.  Every frame
+    equals the per-code learned decode with fallback (posteriors, hard
+    decisions, return value, iterations, fallback flag); the graph takes
+    its frames through bucket lists (the extended tile form), the per-code
+    call through contiguous tiles (the plain form).  A second replay into
+    sentinel-filled outputs is identical."""
+    import json
+    import os
+
+    import wimax_ldpc_lib_amd as W
+    from wimax_ldpc_lib_amd.mixed import MixedDecoder, code_id
+    from tests.test_gpu_mixed import _gather_index
+    torch = gpu
+    F = 524288
+    rates = [0, 1, 2, 3, 5]
+    codes = [code_id(r, 96) for r in rates]
+    tdir = os.path.join(os.path.dirname(W.__file__), "trained")
+    offsets = {r: np.array(json.load(open(os.path.join(tdir, "offsets_r%d_z96_T5.json" % r)))["offsets"],
+                           np.float32) for r in rates}
+    md = MixedDecoder(max_iter=5, early_stop=True, codes=codes)
+    for r in rates:
+        md.set_offsets(r, offsets[r])
+    cfg_np = np.random.default_rng(4242).choice(np.array(codes, dtype=np.uint8), F)
+    cfg = torch.from_numpy(cfg_np).cuda()
+    off = torch.empty(F, dtype=torch.int64, device="cuda")
+    md.offsets_dev(cfg, off)
+    total = 2304 * F
+    llr = torch.empty(total, dtype=torch.float32, device="cuda")
+    md.channel_dev(4242, cfg, off, 1.5, 5.0, llr)
+
+    def run():
+        post = torch.full((total,), float("nan"), dtype=torch.float32, device="cuda")
+        hard = torch.full((total // 8,), 0xAB, dtype=torch.uint8, device="cuda")
+        ret = torch.full((F,), 0xAB, dtype=torch.uint8, device="cuda")
+        iters = torch.full((F,), 0xAB, dtype=torch.uint8, device="cuda")
+        fb = torch.full((F,), 0xAB, dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        md.decode_learned_fallback_dev(cfg, off, llr, 10, post, hard, ret, iters, None, fb)
+        torch.cuda.synchronize()
+        return post.view(torch.int32), hard, ret, iters, fb
+
+    first = run()
+    post, hard, ret, iters, fb = first
+    frac = float(fb.float().mean())
+    assert 0.05 < frac < 0.6, frac
+    offs = off.cpu().numpy()
+    for r in rates:
+        sel = np.flatnonzero(cfg_np == code_id(r, 96))
+        gi = _gather_index(torch, offs, sel, 2304)
+        hi = _gather_index(torch, offs // 8, sel, 288)
+        dec = W.ldpc_decoder(r, 96, 5, early_stop=True)
+        dec.set_offsets(offsets[r])
+        frames = llr[gi].contiguous()
+        p1 = torch.empty_like(frames)
+        h1 = torch.empty((len(sel), 288), dtype=torch.uint8, device="cuda")
+        r1 = torch.empty(len(sel), dtype=torch.uint8, device="cuda")
+        i1 = torch.empty(len(sel), dtype=torch.uint8, device="cuda")
+        f1 = torch.empty(len(sel), dtype=torch.uint8, device="cuda")
+        dec.decode_learned_fallback_dev(frames, 10, p1, h1, r1, i1, None, f1)
+        torch.cuda.synchronize()
+        s = torch.from_numpy(sel).cuda()
+        assert torch.equal(post[gi], p1.view(torch.int32)), r
+        assert torch.equal(hard[hi], h1), r
+        assert torch.equal(ret[s], r1) and torch.equal(iters[s], i1) and torch.equal(fb[s], f1), r
+        del gi, hi, frames, p1
+    second = run()
+    for a, b in zip(first, second):
+        assert torch.equal(a, b)
