@@ -92,22 +92,27 @@ def test_training_reduces_loss_and_resumes(gpu, oracle_mod, tmp_path):
         tr3.load(ck)
 
 
-@pytest.mark.parametrize("z,order", [(96, "split"), (96, "interleaved"), (48, "split"), (24, "interleaved"),
-                                     (32, "interleaved")])
+@pytest.mark.parametrize("z,order", [(96, "split"), (96, "interleaved"), (96, "odd"), (48, "split"),
+                                     (24, "interleaved"), (24, "odd"), (32, "interleaved")])
 def test_fallback_reroutes_failed_frames(gpu, oracle_mod, z, order):
     """Frames the learned pass cannot clean (pure noise) are decoded again by
     plain normalised min-sum from their input LLRs; clean ones are not.
     z = 24 / 96: one persistent kernel re-queueing in LDS; z = 48 / 32:
     learned pass, compaction, normalised min-sum pass over the compacted
-    frame list (at z = 32 with the two-halves code shape the table picks)."""
+    frame list (at z = 32 with the two-halves code shape the table picks).
+    "odd": 31 frames, the last learned tile and the last re-queued tile
+    partly filled."""
     import wimax_ldpc_lib_amd as W
     torch = gpu
     code = oracle_mod.Code(1, z)
+    n_noise = 15 if order == "odd" else 16
     good, _ = noisy_llr(code, 16, 4.0, seed=51)
-    noise = np.random.default_rng(52).normal(0, 1, (16, code.n)).astype(np.float32)
+    noise = np.random.default_rng(52).normal(0, 1, (n_noise, code.n)).astype(np.float32)
     llr = np.concatenate([good, noise])
     if order == "interleaved":
         llr = np.ascontiguousarray(np.stack([llr[i // 2 + (16 if i % 2 else 0)] for i in range(32)]))
+    elif order == "odd":
+        llr = np.ascontiguousarray(np.stack([llr[i // 2 + (16 if i % 2 else 0)] for i in range(30)] + [llr[15]]))
     off = np.full((5, code.mb, code.c), 0.15, np.float32)
     dec = W.ldpc_decoder(1, z, 5, early_stop=True)
     dec.set_offsets(off)
@@ -123,7 +128,7 @@ def test_fallback_reroutes_failed_frames(gpu, oracle_mod, z, order):
     fb = fb.cpu().numpy()
     o_l = code.decode(llr, 5, True, variant=oracle_mod.LEARNED, offsets=off)
     o_n = code.decode(llr, 10, True)
-    assert np.array_equal(fb, o_l["ret"]) and fb.sum() == 16
+    assert np.array_equal(fb, o_l["ret"]) and fb.sum() == n_noise
     want_post = np.where(fb[:, None] == 1, o_n["post"], o_l["post"])
     want_hd = np.where(fb[:, None] == 1, o_n["hd"], o_l["hd"])
     assert bits_equal_f32(post.cpu().numpy(), want_post)
