@@ -92,7 +92,8 @@ int wl_code_table(int rate, int z, int16_t* out, size_t out_len);
 /* Replaces ldpc_decoder::ldpc_decoder(rate, z, max_iter, num_threads)
  * (include/ldpc_decoder.h:25-28, lib/ldpc_decoder.cc:27-59).  flags:
  * WL_FLAG_EARLY_STOP, and at most one of WL_FLAG_OMS / WL_FLAG_MS (the
- * offset and plain rules use the register-lean code shape whatever
+ * offset and plain rules use code shape 2 at two frames per workgroup where
+ * z % 32 == 0 and the register-lean shape otherwise, whatever
  * wl_decoder_set_variant chose; learned offsets, when set, replace the rule,
  * and the learned fallback re-decodes with normalised min-sum; run-time QC
  * codes: WL_ENOTSUP). */
@@ -108,7 +109,8 @@ int wl_decoder_destroy(wl_decoder* dec);
  * holding 32 rows of two frames -- only for z % 32 == 0 and
  * frames_per_wg 2, 3: 16 rows of four frames -- only for z = 32, 48, 64 and
  * frames_per_wg 4; set frames_per_wg first, and another frames_per_wg then
- * switches 2 and 3 to 1).  Results are identical for every shape; create()
+ * switches 2 and 3 to 1; 4: as 0 compiled for four waves per SIMD, for
+ * contiguous batches).  Results are identical for every shape; create()
  * starts from the autotuned table. */
 int wl_decoder_set_frames_per_wg(wl_decoder* dec, int frames_per_wg);
 int wl_decoder_get_frames_per_wg(const wl_decoder* dec, int* out);

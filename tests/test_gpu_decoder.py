@@ -92,7 +92,9 @@ def test_all_codes(gpu, oracle_mod, rate):
 
 @pytest.mark.parametrize("rate,z", [(0, 24), (5, 96), (3, 52), (1, 60)])
 def test_frames_per_workgroup_and_ragged_batches(gpu, oracle_mod, rate, z):
-    """Every frames-per-workgroup setting and batch sizes that leave partial workgroups."""
+    """Every frames-per-workgroup setting, in the code shapes that take any
+    (0: hoisted slot addresses, 1: register-lean, 4: hoisted at 4 waves per
+    SIMD), and batch sizes that leave partial workgroups."""
     import wimax_ldpc_lib_amd as W
     code = oracle_mod.Code(rate, z)
     llr, _ = noisy_llr(code, 37, 2.0 if rate < 3 else 3.0, seed=7 * z + rate)
@@ -109,10 +111,12 @@ def test_frames_per_workgroup_and_ragged_batches(gpu, oracle_mod, rate, z):
     assert fmax >= 2
     for f in range(1, fmax + 1):
         dec.frames_per_wg = f
-        for nf in (1, f, 37):
-            g = _gpu_decode(gpu, dec, np.ascontiguousarray(llr[:nf]))
-            oo = {k: v[:nf] for k, v in o.items() if k != "seconds"}
-            _compare(g, oo, "F=%d nframes=%d" % (f, nf))
+        for variant in (0, 1, 4):
+            dec.variant = variant
+            for nf in (1, f, 37):
+                g = _gpu_decode(gpu, dec, np.ascontiguousarray(llr[:nf]))
+                oo = {k: v[:nf] for k, v in o.items() if k != "seconds"}
+                _compare(g, oo, "F=%d variant %d nframes=%d" % (f, variant, nf))
 
 
 def test_edge_cases(gpu, oracle_mod):

@@ -3,8 +3,8 @@
 For every code it times the normalised min-sum kernel (exactly `--iters`
 iterations, no early stop, so the work is the same for every shape) over
 each code-shape variant and every frames-per-workgroup value the kernel
-accepts (shape 2, where z % 32 == 0, at two frames per workgroup only;
-shape 3, at z = 32, 48, 64, at four),
+accepts (shapes 0, 1 and 4 at any; shape 2, where z % 32 == 0, at two
+frames per workgroup only; shape 3, at z = 32, 48, 64, at four),
 on a batch of seeded channel frames holding about 150 Mbit of
 codeword bits, in three interleaved sweeps (each shape keeps its fastest),
 and keeps the fastest shape.  Every shape is checked to give the same hard
@@ -49,7 +49,7 @@ def tune_one(torch, W, rate, z, iters, reps, bits_target, rounds=3):
     ret = torch.empty(F, dtype=torch.uint8, device="cuda")
     ref = None
     results = []
-    dec.variant = 0   # any frames-per-workgroup value is allowed with shapes 0 and 1
+    dec.variant = 0   # any frames-per-workgroup value is allowed with shapes 0, 1 and 4
     fmax = 1
     while True:
         try:
@@ -57,7 +57,7 @@ def tune_one(torch, W, rate, z, iters, reps, bits_target, rounds=3):
             fmax += 1
         except W.WLError:
             break
-    shapes = [(v, f) for v in (0, 1) for f in range(1, fmax + 1)]
+    shapes = [(v, f) for v in (0, 1, 4) for f in range(1, fmax + 1)]
     if z % 32 == 0 and fmax >= 2:
         shapes.append((2, 2))   # two-halves lane mapping: two frames per workgroup only
     if z % 16 == 0 and z <= 64 and fmax >= 4:

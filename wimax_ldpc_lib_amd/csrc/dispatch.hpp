@@ -14,8 +14,10 @@ using DecodeOccupancy = int (*)(int frames_per_wg);
 // code-shape variants of the NMS kernel (tdmp.hpp VAR_*): 0 hoisted row
 // addresses, 1 VAR_LEAN, 2 VAR_LEAN_W2 (two frames per workgroup, z % 32 == 0
 // only), 3 VAR_LEAN_W4 (four frames per workgroup, z = 32, 48, 64 only: one
-// loop copy per wave, at most four); null where a shape does not exist
-constexpr int kNumVariants = 4;
+// loop copy per wave, at most four), 4 the hoisted shape compiled for 4 waves
+// per SIMD (contiguous batches; frame lists and persistent launches take
+// variant 0); null where a shape does not exist
+constexpr int kNumVariants = 5;
 constexpr int kVariantW2 = 2;
 constexpr int kVariantW4 = 3;
 
@@ -103,6 +105,15 @@ hipError_t launch_learned(const DecodeArgs& a, hipStream_t stream)
     return launch_tdmp<RATE, Z, MODE_LEARNED, 0>(a, stream);
 }
 
+// variant 4: contiguous batches in the hoisted shape at 4 waves per SIMD,
+// everything else in variant 0's extended kernel
+template <int RATE, int Z>
+hipError_t launch_hoist4(const DecodeArgs& a, hipStream_t stream)
+{
+    if (a.frame_idx || a.bucket || a.frame_off || a.tile_counter) return launch_tdmp<RATE, Z, MODE_NMS, 0>(a, stream);
+    return launch_tdmp_plain<RATE, Z, MODE_NMS, VAR_HOIST4>(a, stream);
+}
+
 // offset / plain min-sum: the two-halves shape where the decoder runs two
 // frames per workgroup and z % 32 == 0 (as the other rules there), the
 // register-lean shape otherwise
@@ -150,12 +161,14 @@ constexpr auto make_decode_table(std::integer_sequence<int, ZI...>)
     struct Arr { DecodeEntry e[sizeof...(ZI)]; };
     return Arr{{DecodeEntry{{&launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0>,
                              &launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN>,
-                             w2_launcher<RATE, z_of_index(ZI)>(), w4_launcher<RATE, z_of_index(ZI)>()},
+                             w2_launcher<RATE, z_of_index(ZI)>(), w4_launcher<RATE, z_of_index(ZI)>(),
+                             &launch_hoist4<RATE, z_of_index(ZI)>},
                             &launch_learned<RATE, z_of_index(ZI)>,
                             max_frames_per_wg<RATE, z_of_index(ZI)>(),
                             {&occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0, true>,
                              &occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN, true>,
-                             w2_occupancy<RATE, z_of_index(ZI)>(), w4_occupancy<RATE, z_of_index(ZI)>()},
+                             w2_occupancy<RATE, z_of_index(ZI)>(), w4_occupancy<RATE, z_of_index(ZI)>(),
+                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0, true>},
                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_LEARNED, 0, true>,
                             fb_launcher<RATE, z_of_index(ZI)>(),
                             fb_occupancy<RATE, z_of_index(ZI)>(),

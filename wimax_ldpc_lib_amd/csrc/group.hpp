@@ -36,7 +36,8 @@ struct GroupShape {
     // or 3), whose F is fixed by the shape (if all were, F = 2)
     static constexpr int rate_f(int r)
     {
-        return tuned_variant(r, Z) >= 2 ? 0 : (tuned_f(r, Z) > 0 ? tuned_f(r, Z) : 1);
+        const int v = tuned_variant(r, Z);
+        return (v == 2 || v == 3) ? 0 : (tuned_f(r, Z) > 0 ? tuned_f(r, Z) : 1);
     }
     static constexpr int pick_f()
     {

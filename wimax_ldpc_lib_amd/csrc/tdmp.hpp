@@ -157,11 +157,16 @@ enum { MODE_NMS = 0, MODE_LEARNED = 1, MODE_LEARNED_FB = 2, MODE_ALT = 3 };
 //   VAR_W4       the same with four frames per workgroup and z % 16 == 0:
 //                lane l of wave w owns row 16 w + l % 16 of frame l / 16
 //                (frames 16 floats apart beyond n)
-enum { VAR_NOHOIST = 1, VAR_REREAD = 2, VAR_SEL = 4, VAR_SMASK = 8, VAR_W2 = 16, VAR_W4 = 32 };
+//   VAR_OCC4     no change to the code: the plain (contiguous) kernel is
+//                compiled for 4 waves per SIMD (128 registers, spilling what
+//                does not fit) instead of the registers the shape takes
+enum { VAR_NOHOIST = 1, VAR_REREAD = 2, VAR_SEL = 4, VAR_SMASK = 8, VAR_W2 = 16, VAR_W4 = 32, VAR_OCC4 = 64 };
 // the low-register code shape of the tuned variants
 constexpr int VAR_LEAN = VAR_NOHOIST | VAR_SEL | VAR_SMASK;
 constexpr int VAR_LEAN_W2 = VAR_LEAN | VAR_W2;
 constexpr int VAR_LEAN_W4 = VAR_LEAN | VAR_W4;
+// the hoisted-address shape compiled for 4 waves per SIMD
+constexpr int VAR_HOIST4 = VAR_OCC4;
 
 // rows of one frame per wave in the wave-split shapes (0: not split); the
 // workgroup then decodes 64 / rows frames
@@ -720,11 +725,16 @@ __device__ __forceinline__ void tdmp_tile(const DecodeArgs& a, unsigned char* sm
 // the wave-split shapes (their register budget then drops to 128, with a few
 // spills outside the layer loop; 2.4 % faster at rate 5/6 z=96 than the 3
 // waves their natural 132-133 registers give) and for the learned +
-// re-queue kernel of those shapes, no constraint otherwise.
+// re-queue kernel of those shapes; 4 for the plain kernels of the
+// register-lean shape (111-139 registers unconstrained; never slower at 4
+// waves, 4-7 % faster for the high-rate codes at their best frame counts)
+// and of VAR_OCC4 shapes; no constraint otherwise.
 template <int MODE, int VAR, bool EXT>
 constexpr int min_waves_per_simd()
 {
-    return (split_rows<VAR>() != 0 && (!EXT || MODE == MODE_LEARNED_FB)) ? 4 : 1;
+    if (split_rows<VAR>() != 0 && (!EXT || MODE == MODE_LEARNED_FB)) return 4;
+    if (!EXT && (VAR == VAR_LEAN || (VAR & VAR_OCC4) != 0)) return 4;
+    return 1;
 }
 
 template <int RATE, int Z, int MODE, int VAR = 0, bool EXT = false>
@@ -838,6 +848,23 @@ int occupancy_tdmp(int F)
         hipSuccess)
         return 0;
     return nb;
+}
+
+// A plain (contiguous, one tile per workgroup) launch of a shape compiled in
+// the plain form only.
+template <int RATE, int Z, int MODE, int VAR>
+hipError_t launch_tdmp_plain(const DecodeArgs& a, hipStream_t stream)
+{
+    const int F = a.frames_per_wg;
+    if (F < 1 || F > max_frames_per_wg<RATE, Z>()) return hipErrorInvalidValue;
+    if (a.frame_idx || a.bucket || a.frame_off || a.tile_counter) return hipErrorInvalidValue;
+    if (a.nframes == 0) return hipSuccess;
+    const int threads = ((F * Z + 63) / 64) * 64;
+    const size_t lds = kLdsPad + (size_t)F * lds_frame_stride<Code<RATE, Z>::N, VAR>() * sizeof(float);
+    const unsigned long long grid = (a.nframes + F - 1) / F;
+    if (grid > 0x7fffffffull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((tdmp_kernel<RATE, Z, MODE, VAR, false>), dim3((unsigned)grid), dim3(threads), lds, stream, a);
+    return hipGetLastError();
 }
 
 // EXT kernels serve indirect, ragged and persistent launches; the plain one
