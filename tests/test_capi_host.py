@@ -146,8 +146,8 @@ def test_header_flag_values_match_bindings():
 def test_tuned_table_is_valid():
     """The autotuned launch table compiled into the library covers all 114
     codes once, with frames per workgroup the kernel accepts and each code
-    shape only where it exists (2: z % 32 == 0 at two frames per workgroup,
-    3: z = 32, 48, 64 at four)."""
+    shape only where it exists (0, 1 and 4 at any frame count; 2: z % 32 == 0
+    at two frames per workgroup, 3: z = 32, 48, 64 at four)."""
     path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "wimax_ldpc_lib_amd",
                         "csrc", "tuned_table.inc")
     rows = [tuple(int(x) for x in re.findall(r"-?\d+", line)) for line in open(path) if line.startswith("{")]
@@ -157,7 +157,7 @@ def test_tuned_table_is_valid():
         while fmax > 1 and 512 + fmax * 24 * z * 4 > 65536:
             fmax -= 1
         assert 1 <= f <= fmax, (rate, z, f)
-        assert v in (0, 1, 2, 3), (rate, z, v)
+        assert v in (0, 1, 2, 3, 4), (rate, z, v)
         if v == 2:
             assert z % 32 == 0 and f == 2, (rate, z, f)
         if v == 3:
