@@ -34,11 +34,13 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include")]
 
-HIP_SOURCES = ["decode_r0.hip", "decode_r1.hip", "decode_r2.hip", "decode_r3.hip", "decode_r4.hip",
-               "decode_r5.hip", "encoder.hip", "capi.hip", "learned.hip", "trainer.hip", "mixed.hip",
-               "generic.hip", "qc_host.hip", "decode_g0.hip", "decode_g1.hip", "decode_g2.hip", "decode_g3.hip",
-               "decode_g4.hip", "decode_p0.hip", "decode_p1.hip", "decode_p2.hip", "decode_p3.hip", "decode_p4.hip",
-               "decode_p5.hip"]
+# longest compiles first, so that a parallel build keeps every worker busy
+HIP_SOURCES = ["decode_r0_lo.hip", "decode_r0_hi.hip", "decode_r1_lo.hip", "decode_r1_hi.hip",
+               "decode_r2_lo.hip", "decode_r2_hi.hip", "decode_r3_lo.hip", "decode_r3_hi.hip",
+               "decode_r4_lo.hip", "decode_r4_hi.hip", "decode_r5_lo.hip", "decode_r5_hi.hip",
+               "decode_g0.hip", "decode_g1.hip", "decode_g2.hip", "decode_g3.hip", "decode_g4.hip",
+               "capi.hip", "mixed.hip", "learned.hip", "trainer.hip", "encoder.hip", "generic.hip", "qc_host.hip",
+               "decode_p0.hip", "decode_p1.hip", "decode_p2.hip", "decode_p3.hip", "decode_p4.hip", "decode_p5.hip"]
 CXX_SOURCES = ["ldpc_classes.cpp", "matrices.cpp"]
 BINDIR = os.path.join(PKG, "bin")
 # host programs linked against the library: (source, output, rpath relative to the output)

@@ -1,0 +1,3 @@
+// decode_r3_hi.hip -- TDMP decoder kernels for coderate 3 at lifting factors 64..96.
+#include "dispatch.hpp"
+WL_DEFINE_DECODE_TABLE_hi(3)

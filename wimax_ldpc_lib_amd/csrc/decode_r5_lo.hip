@@ -1,0 +1,3 @@
+// decode_r5_lo.hip -- TDMP decoder kernels for coderate 5 at lifting factors 24..60.
+#include "dispatch.hpp"
+WL_DEFINE_DECODE_TABLE_lo(5)

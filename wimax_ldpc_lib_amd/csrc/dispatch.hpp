@@ -132,56 +132,78 @@ constexpr DecodeOccupancy fb_occupancy()
     else return nullptr;
 }
 
-// One table of kNumZ entries per rate, defined in decode_r<rate>.hip.
-const DecodeEntry* decode_table_r0();
-const DecodeEntry* decode_table_r1();
-const DecodeEntry* decode_table_r2();
-const DecodeEntry* decode_table_r3();
-const DecodeEntry* decode_table_r4();
-const DecodeEntry* decode_table_r5();
+// Each rate's kNumZ entries come in two parts, compiled in two units so
+// that a parallel build spreads the kernels: lifting sizes 24..60 (z index
+// 0..kZSplit-1) in decode_r<rate>_lo.hip, 64..96 in decode_r<rate>_hi.hip.
+constexpr int kZSplit = 10;
+const DecodeEntry* decode_table_r0_lo();
+const DecodeEntry* decode_table_r1_lo();
+const DecodeEntry* decode_table_r2_lo();
+const DecodeEntry* decode_table_r3_lo();
+const DecodeEntry* decode_table_r4_lo();
+const DecodeEntry* decode_table_r5_lo();
+const DecodeEntry* decode_table_r0_hi();
+const DecodeEntry* decode_table_r1_hi();
+const DecodeEntry* decode_table_r2_hi();
+const DecodeEntry* decode_table_r3_hi();
+const DecodeEntry* decode_table_r4_hi();
+const DecodeEntry* decode_table_r5_hi();
 
 inline const DecodeEntry* decode_entry(int rate, int z)
 {
     if (rate < 0 || rate >= kNumRates || !valid_z(z)) return nullptr;
+    const int zi = index_of_z(z);
+    const bool hi = zi >= kZSplit;
     const DecodeEntry* t = nullptr;
     switch (rate) {
-    case 0: t = decode_table_r0(); break;
-    case 1: t = decode_table_r1(); break;
-    case 2: t = decode_table_r2(); break;
-    case 3: t = decode_table_r3(); break;
-    case 4: t = decode_table_r4(); break;
-    default: t = decode_table_r5(); break;
+    case 0: t = hi ? decode_table_r0_hi() : decode_table_r0_lo(); break;
+    case 1: t = hi ? decode_table_r1_hi() : decode_table_r1_lo(); break;
+    case 2: t = hi ? decode_table_r2_hi() : decode_table_r2_lo(); break;
+    case 3: t = hi ? decode_table_r3_hi() : decode_table_r3_lo(); break;
+    case 4: t = hi ? decode_table_r4_hi() : decode_table_r4_lo(); break;
+    default: t = hi ? decode_table_r5_hi() : decode_table_r5_lo(); break;
     }
-    return &t[index_of_z(z)];
+    return &t[hi ? zi - kZSplit : zi];
 }
 
-template <int RATE, int... ZI>
-constexpr auto make_decode_table(std::integer_sequence<int, ZI...>)
+// entries of z index LO + i for i in I...
+template <int RATE, int LO, int... I>
+constexpr auto make_decode_table(std::integer_sequence<int, I...>)
 {
-    struct Arr { DecodeEntry e[sizeof...(ZI)]; };
-    return Arr{{DecodeEntry{{&launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0>,
-                             &launch_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN>,
-                             w2_launcher<RATE, z_of_index(ZI)>(), w4_launcher<RATE, z_of_index(ZI)>(),
-                             &launch_hoist4<RATE, z_of_index(ZI)>},
-                            &launch_learned<RATE, z_of_index(ZI)>,
-                            max_frames_per_wg<RATE, z_of_index(ZI)>(),
-                            {&occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0, true>,
-                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, VAR_LEAN, true>,
-                             w2_occupancy<RATE, z_of_index(ZI)>(), w4_occupancy<RATE, z_of_index(ZI)>(),
-                             &occupancy_tdmp<RATE, z_of_index(ZI), MODE_NMS, 0, true>},
-                            &occupancy_tdmp<RATE, z_of_index(ZI), MODE_LEARNED, 0, true>,
-                            fb_launcher<RATE, z_of_index(ZI)>(),
-                            fb_occupancy<RATE, z_of_index(ZI)>(),
-                            &launch_alt<RATE, z_of_index(ZI)>}...}};
+    struct Arr { DecodeEntry e[sizeof...(I)]; };
+    return Arr{{DecodeEntry{{&launch_tdmp<RATE, z_of_index(LO + I), MODE_NMS, 0>,
+                             &launch_tdmp<RATE, z_of_index(LO + I), MODE_NMS, VAR_LEAN>,
+                             w2_launcher<RATE, z_of_index(LO + I)>(), w4_launcher<RATE, z_of_index(LO + I)>(),
+                             &launch_hoist4<RATE, z_of_index(LO + I)>},
+                            &launch_learned<RATE, z_of_index(LO + I)>,
+                            max_frames_per_wg<RATE, z_of_index(LO + I)>(),
+                            {&occupancy_tdmp<RATE, z_of_index(LO + I), MODE_NMS, 0, true>,
+                             &occupancy_tdmp<RATE, z_of_index(LO + I), MODE_NMS, VAR_LEAN, true>,
+                             w2_occupancy<RATE, z_of_index(LO + I)>(), w4_occupancy<RATE, z_of_index(LO + I)>(),
+                             &occupancy_tdmp<RATE, z_of_index(LO + I), MODE_NMS, 0, true>},
+                            &occupancy_tdmp<RATE, z_of_index(LO + I), MODE_LEARNED, 0, true>,
+                            fb_launcher<RATE, z_of_index(LO + I)>(),
+                            fb_occupancy<RATE, z_of_index(LO + I)>(),
+                            &launch_alt<RATE, z_of_index(LO + I)>}...}};
 }
 
 }  // namespace wl
 
-#define WL_DEFINE_DECODE_TABLE(RATE)                                                        \
+// PART lo: z index 0 .. kZSplit-1; PART hi: kZSplit .. kNumZ-1
+#define WL_DEFINE_DECODE_TABLE_lo(RATE)                                                     \
     namespace wl {                                                                          \
-    const DecodeEntry* decode_table_r##RATE()                                               \
+    const DecodeEntry* decode_table_r##RATE##_lo()                                          \
     {                                                                                       \
-        static const auto tab = make_decode_table<RATE>(std::make_integer_sequence<int, kNumZ>{}); \
+        static const auto tab = make_decode_table<RATE, 0>(std::make_integer_sequence<int, kZSplit>{}); \
+        return tab.e;                                                                       \
+    }                                                                                       \
+    }
+#define WL_DEFINE_DECODE_TABLE_hi(RATE)                                                     \
+    namespace wl {                                                                          \
+    const DecodeEntry* decode_table_r##RATE##_hi()                                          \
+    {                                                                                       \
+        static const auto tab =                                                             \
+            make_decode_table<RATE, kZSplit>(std::make_integer_sequence<int, kNumZ - kZSplit>{}); \
         return tab.e;                                                                       \
     }                                                                                       \
     }
