@@ -93,8 +93,9 @@ int wl_code_table(int rate, int z, int16_t* out, size_t out_len);
  * (include/ldpc_decoder.h:25-28, lib/ldpc_decoder.cc:27-59).  flags:
  * WL_FLAG_EARLY_STOP, and at most one of WL_FLAG_OMS / WL_FLAG_MS (the
  * offset and plain rules use code shape 2 at two frames per workgroup where
- * z % 32 == 0 and the register-lean shape otherwise, whatever
- * wl_decoder_set_variant chose; learned offsets, when set, replace the rule,
+ * z % 32 == 0 and the register-lean shape otherwise, at 4 waves per SIMD,
+ * whatever wl_decoder_set_variant chose; learned offsets, when set, replace
+ * the rule,
  * and the learned fallback re-decodes with normalised min-sum; run-time QC
  * codes: WL_ENOTSUP). */
 int wl_decoder_create(int rate, int z, int max_iter, unsigned flags, wl_decoder** out);

@@ -882,9 +882,8 @@ hipError_t launch_tdmp(const DecodeArgs& a, hipStream_t stream)
     if (a.tile_counter && a.persistent_grid > 0 && (unsigned long long)a.persistent_grid < grid)
         grid = (unsigned long long)a.persistent_grid;
     if (grid > 0x7fffffffull) return hipErrorInvalidValue;
-    // the alternative rules of the register-lean shape are compiled in the
-    // extended form only (it also decodes contiguous batches)
-    constexpr bool ext_only = MODE == MODE_LEARNED_FB || (MODE == MODE_ALT && split_rows<VAR>() == 0);
+    // the learned + re-queue kernel is persistent: extended form only
+    constexpr bool ext_only = MODE == MODE_LEARNED_FB;
     const bool ext = ext_only || a.frame_idx || a.bucket || a.frame_off || a.tile_counter;
     if (ext)
         hipLaunchKernelGGL((tdmp_kernel<RATE, Z, MODE, VAR, true>), dim3((unsigned)grid), dim3(threads), lds, stream, a);
