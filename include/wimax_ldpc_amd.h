@@ -96,8 +96,7 @@ int wl_code_table(int rate, int z, int16_t* out, size_t out_len);
  * z % 32 == 0 and the register-lean shape otherwise, at 4 waves per SIMD,
  * whatever wl_decoder_set_variant chose; learned offsets, when set, replace
  * the rule, and the learned fallback re-decodes with normalised min-sum;
- * run-time QC
- * codes: WL_ENOTSUP). */
+ * run-time QC codes: WL_ENOTSUP). */
 int wl_decoder_create(int rate, int z, int max_iter, unsigned flags, wl_decoder** out);
 int wl_decoder_destroy(wl_decoder* dec);
 
