@@ -90,6 +90,31 @@ def test_all_codes(gpu, oracle_mod, rate):
                 _compare(_gpu_decode(gpu, dec, llr), o, "rate %d z %d early %d small_batch %r" % (rate, z, early, small))
 
 
+@pytest.mark.parametrize("rate", range(6))
+def test_every_code_in_every_any_frame_shape(gpu, oracle_mod, rate):
+    """Every (rate, z) in the code shapes that take any frame count (0, 1,
+    and 4: the hoisted shape compiled for 4 waves per SIMD), at the code's
+    tuned frames per workgroup, both stop modes, an odd batch (a partly
+    filled last workgroup), the row-per-lane kernel."""
+    import wimax_ldpc_lib_amd as W
+    ebn0 = {0: 1.8, 1: 2.4, 2: 2.4, 3: 2.8, 4: 2.8, 5: 3.4}[rate]
+    for z in Z_VALUES:
+        code = oracle_mod.Code(rate, z)
+        if rate == 4:
+            llr = np.random.default_rng(300 + z).normal(1.0, 2.0, (11, code.n)).astype(np.float32)
+        else:
+            llr, _ = noisy_llr(code, 11, ebn0, seed=5000 * rate + z)
+        for early in (True, False):
+            o = code.decode(llr, max_iter=7, early_stop=early)
+            dec = W.ldpc_decoder(rate, z, 7, early_stop=early)
+            dec.small_batch = 0
+            f = dec.frames_per_wg
+            for variant in (0, 1, 4):
+                dec.variant = variant
+                _compare(_gpu_decode(gpu, dec, llr), o,
+                         "rate %d z %d early %d F %d variant %d" % (rate, z, early, f, variant))
+
+
 @pytest.mark.parametrize("rate,z", [(0, 24), (5, 96), (3, 52), (1, 60)])
 def test_frames_per_workgroup_and_ragged_batches(gpu, oracle_mod, rate, z):
     """Every frames-per-workgroup setting, in the code shapes that take any
