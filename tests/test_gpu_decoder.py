@@ -479,3 +479,47 @@ def test_offset_and_plain_min_sum_rules(gpu, oracle_mod, rule, rate):
             dec = W.ldpc_decoder(rate, 96, max_iter, early_stop=True, rule=rule)
             _compare(_gpu_decode(gpu, dec, llr), o, "%s special max_iter %d" % (rule, max_iter))
 
+
+
+@pytest.mark.parametrize("rate,z,F", [(5, 96, 4096), (0, 24, 64)])
+def test_decode_recorded_in_a_graph(gpu, oracle_mod, rate, z, F):
+    """wl_decode_f32_dev allocates nothing for device buffers, so a serving
+    loop can record it into a HIP graph (torch.cuda.graph, side stream) and
+    replay it on new inputs written into the captured buffer: three replays
+    with eager work between them, each equal to an eager decode (the row-
+    per-lane kernel at 4096 frames, the small-batch kernel at 64)."""
+    import wimax_ldpc_lib_amd as W
+    torch = gpu
+    code = oracle_mod.Code(rate, z)
+    dec = W.ldpc_decoder(rate, z, 10, early_stop=True)
+    llr = torch.empty((F, code.n), dtype=torch.float32, device="cuda")
+    post = torch.empty_like(llr)
+    hard = torch.empty((F, code.n // 8), dtype=torch.uint8, device="cuda")
+    ret = torch.empty(F, dtype=torch.uint8, device="cuda")
+    iters = torch.empty(F, dtype=torch.uint8, device="cuda")
+    enc = W.ldpc_encoder(rate, z)
+    enc.channel_dev(11, 0, F, 3.0, llr=llr)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        dec.decode_dev(llr, post, hard, ret, iters, stream=side.cuda_stream)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        dec.decode_dev(llr, post, hard, ret, iters, stream=side.cuda_stream)
+    torch.cuda.synchronize()
+    for seed in (21, 22, 23):
+        enc.channel_dev(seed, 0, F, 2.0 + 0.5 * (seed - 21), llr=llr)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        got = [t.clone() for t in (post, hard, ret, iters)]
+        p2, h2 = torch.empty_like(post), torch.empty_like(hard)
+        r2, i2 = torch.empty_like(ret), torch.empty_like(iters)
+        dec.decode_dev(llr, p2, h2, r2, i2)
+        torch.cuda.synchronize()
+        assert torch.equal(got[0].view(torch.int32), p2.view(torch.int32)), seed
+        assert torch.equal(got[1], h2) and torch.equal(got[2], r2) and torch.equal(got[3], i2), seed
+    sample = llr[:8].cpu().numpy()
+    o = code.decode(sample, 10, True)
+    assert np.array_equal(hard[:8].cpu().numpy(), pack_bits(o["hd"]))
